@@ -1,0 +1,205 @@
+"""Benchmark: LUCJ applications/s at norb=16, nelec=(8,8) (complex128) on one MI355X.
+
+A step is one application of a spin-balanced LUCJ operator with n_reps=4
+(local interaction pairs aa=(p,p+1), ab=(p,p)) to a resident 12870 x 12870
+complex128 state (2.65 GB), i.e. ``UCJOpSpinBalanced._apply_unitary_``
+(python/ffsim/variational/ucj_spin_balanced.py:657-696) run by the GPU
+library.  Inputs are synthetic (seeded) and resident in HBM before the timed
+region starts.
+
+Printed: one JSON line with the throughput, the roofline of the dominant
+kernel (the fused Givens fiber kernel; durations from HIP events recorded on
+its stream during the timed region) and a CPU baseline measured on this
+host's cores with the oracle's restatement of ffsim's CPU path.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def lucj_pairs(norb: int):
+    pairs_aa = [(p, p + 1) for p in range(norb - 1)]
+    pairs_ab = [(p, p) for p in range(norb)]
+    return pairs_aa, pairs_ab
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> dict:
+    """Time a bounded sample of the same workload with the CPU oracle.
+
+    Sample: one alpha-sector orbital rotation (the 120 Givens rotations and 16
+    phase shifts of ffsim's schedule), one transpose copy and one diag-Coulomb
+    evolution on the norb=16 state.  An application costs 10 sector rotations,
+    10 transpose copies and 4 diag-Coulomb passes on that schedule, so
+    apps/s = 1 / (10 T_rot + 10 T_transpose + 4 T_dc).
+    """
+    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
+    from oracle import oracle as O
+
+    O.build()
+    O.init_cache(norb, nelec)
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    v = np.ascontiguousarray(vec_host.reshape(da, db))
+    U = op.orbital_rotations[0]
+    rots, phases = O.givens_decomposition(U)
+    t0 = time.perf_counter()
+    for c, s, i, j in rots:
+        O._apply_adjacent(v, c, s.conjugate(), (i, j), norb, nelec[0])
+    for i, ph in enumerate(phases):
+        O._phase_shift_in_place(v, complex(ph), O.one_subspace_indices(norb, nelec[0], (i,)))
+    t_rot = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vt = O._transpose_copy(v)
+    t_tr = time.perf_counter() - t0
+    del vt
+    j_aa, j_ab = op.diag_coulomb_mats[0]
+    t0 = time.perf_counter()
+    O.apply_diag_coulomb_evolution(v.reshape(-1), (j_aa, j_ab, j_aa), -1.0, norb, nelec, copy=False)
+    t_dc = time.perf_counter() - t0
+    t_app = 10 * t_rot + 10 * t_tr + 4 * t_dc
+    log(f"cpu sample: sector rotation {t_rot:.2f}s transpose {t_tr:.2f}s diag-coulomb {t_dc:.2f}s")
+    return {
+        "value": 1.0 / t_app,
+        "unit": "LUCJ applications/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (
+            f"C restatement of ffsim's Rust kernels with ffsim's schedule (one FFI call per Givens "
+            f"rotation, per-call threads, transpose copies), {threads} threads: one alpha-sector "
+            f"orbital rotation ({t_rot:.2f} s), one transpose copy ({t_tr:.2f} s) and one "
+            f"diag-Coulomb evolution ({t_dc:.2f} s) of the norb=16 (8,8) state, scaled to one "
+            f"n_reps=4 application = 10 rotations + 10 transposes + 4 diag-Coulomb passes"
+        ),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--norb", type=int, default=16)
+    ap.add_argument("--n-reps", type=int, default=4)
+    ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only: this build has no multi-GPU path "
+            f"(asked for --gpus {args.gpus}).",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd import _lib
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    dev = torch.device("cuda:0")
+    norb = args.norb
+    nelec = (norb // 2, norb // 2)
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    cdtype = torch.complex128 if args.dtype == "c128" else torch.complex64
+    op = ffsim_amd.random.random_ucj_op_spin_balanced(
+        norb, n_reps=args.n_reps, interaction_pairs=lucj_pairs(norb), seed=1234
+    )
+    log(f"workload: LUCJ n_reps={args.n_reps} norb={norb} nelec={nelec} dim={da}x{db} {args.dtype}")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    state = torch.randn(da * db, dtype=cdtype, device=dev, generator=gen)
+    state /= torch.linalg.vector_norm(state)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        op.apply_in_place(state, norb, nelec)
+    torch.cuda.synchronize()
+
+    out = (ctypes.c_double * 6)()
+    _lib.call("ffsim_amd_profile_begin")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.apply_in_place(state, norb, nelec)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.call("ffsim_amd_profile_end", out)
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    fiber_ms, fiber_n, fiber_bytes, diag_ms, diag_n, diag_bytes = list(out)
+    norm = float(torch.linalg.vector_norm(state).item())
+    log(f"timed {args.steps} steps in {elapsed:.3f}s -> {value:.3f} app/s; |psi|={norm:.12f}")
+    log(f"fiber kernel: {fiber_n:.0f} launches {fiber_ms:.2f} ms {fiber_bytes / 1e9:.1f} GB; "
+        f"diag kernel: {diag_n:.0f} launches {diag_ms:.2f} ms {diag_bytes / 1e9:.1f} GB")
+
+    avg_ms = fiber_ms / max(fiber_n, 1)
+    bytes_per_launch = fiber_bytes / max(fiber_n, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "fiber_kernel (fused Givens window pass)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4),
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "launches_per_step": fiber_n / args.steps,
+    }
+    result = {
+        "metric": "LUCJ applies/s + Givens HBM GB/s at norb=16 (8,8); served p50/p99 latency",
+        "value": round(value, 4),
+        "unit": "LUCJ applications/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded random state and LUCJ parameters)",
+        "config": {
+            "workload": f"LUCJ n_reps={args.n_reps} apply_unitary, norb={norb} nelec={nelec}, "
+            f"state {da}x{db} {args.dtype}, interaction pairs aa=(p,p+1) ab=(p,p)",
+            "norb": norb,
+            "nelec": list(nelec),
+            "n_reps": args.n_reps,
+        },
+        "roofline": roofline,
+        "diag_kernel": {
+            "avg_launch_ms": round(diag_ms / max(diag_n, 1), 4),
+            "achieved_GBs": round(diag_bytes / max(diag_ms, 1e-9) / 1e6, 1),
+        },
+    }
+    if not args.no_cpu_baseline:
+        vec_host = state.to(torch.complex128).cpu().numpy()
+        result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, args.cpu_threads)
+        result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,35 @@
+"""ffsim_amd: MI355X (gfx950) drop-ins for ffsim's gate-application hot path.
+
+The public names follow ffsim's (qiskit-community/ffsim): gate functions,
+protocol dispatch, the spin-balanced UCJ operator and the Hamiltonians whose
+linear operators are on the path.  States are numpy arrays (copied to HBM
+and back) or torch tensors on the GPU (kept resident).
+"""
+
+from ffsim_amd import contract, linalg, random
+from ffsim_amd.gates import (
+    apply_diag_coulomb_evolution,
+    apply_num_op_sum_evolution,
+    apply_orbital_rotation,
+)
+from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedHamiltonian
+from ffsim_amd.protocols import apply_unitary, linear_operator
+from ffsim_amd.states import dim, dims, hartree_fock_state
+from ffsim_amd.variational import UCJOpSpinBalanced
+
+__all__ = [
+    "DiagonalCoulombHamiltonian",
+    "DoubleFactorizedHamiltonian",
+    "UCJOpSpinBalanced",
+    "apply_diag_coulomb_evolution",
+    "apply_num_op_sum_evolution",
+    "apply_orbital_rotation",
+    "apply_unitary",
+    "contract",
+    "dim",
+    "dims",
+    "hartree_fock_state",
+    "linalg",
+    "linear_operator",
+    "random",
+]

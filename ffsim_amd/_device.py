@@ -1,0 +1,110 @@
+"""Device-state plumbing: numpy / torch inputs onto HBM and back.
+
+States live in HBM as 1-D torch tensors (complex128 or complex64) viewed by
+the kernels as dim_a x dim_b row-major matrices.  numpy inputs are copied to
+the device and the result is copied back; torch tensors already on the GPU
+stay resident and are returned resident.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Callable
+
+import numpy as np
+import torch
+
+from ffsim_amd import _lib
+
+
+def require_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "ffsim_amd gate kernels need a ROCm GPU (torch.cuda.is_available() is False); "
+            "there is no CPU fallback"
+        )
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def dtype_tag(t: torch.Tensor) -> int:
+    if t.dtype == torch.complex128:
+        return _lib.C128
+    if t.dtype == torch.complex64:
+        return _lib.C64
+    raise TypeError(f"state dtype must be complex128 or complex64, got {t.dtype}")
+
+
+def ptr(t: torch.Tensor | None):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def host_mat(m, shape=None, dtype=np.complex128):
+    """C-contiguous host array (or None) for the C ABI."""
+    if m is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(m), dtype=dtype)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def host_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+@dataclass
+class DeviceState:
+    """A working 1-D device tensor plus the conversion back to the caller's type."""
+
+    tensor: torch.Tensor
+    restore: Callable[[torch.Tensor], object]
+
+
+def to_device(vec, copy: bool) -> DeviceState:
+    if isinstance(vec, torch.Tensor):
+        if vec.device.type != "cuda":
+            host = vec.detach().numpy()
+            return to_device(host, copy=True)
+        t = vec.reshape(-1)
+        if t.dtype not in (torch.complex128, torch.complex64):
+            t = t.to(torch.complex128)
+        elif copy:
+            t = t.clone()
+        if not t.is_contiguous():
+            t = t.contiguous()
+        return DeviceState(t, lambda out: out)
+    dev = require_device()
+    arr = np.asarray(vec)
+    if arr.dtype not in (np.complex128, np.complex64):
+        arr = arr.astype(np.complex128)
+    t = torch.from_numpy(np.ascontiguousarray(arr).reshape(-1)).to(dev)
+    return DeviceState(t, lambda out: out.cpu().numpy())
+
+
+def dims(norb: int, nelec) -> tuple[int, int]:
+    if isinstance(nelec, int):
+        return math.comb(norb, nelec), 1
+    return math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+
+
+def check_size(t: torch.Tensor, norb: int, nelec) -> None:
+    da, db = dims(norb, nelec)
+    if t.numel() != da * db:
+        raise ValueError(
+            f"state has {t.numel()} entries but norb={norb}, nelec={nelec} needs {da * db}"
+        )
+
+
+def work_buffer(t: torch.Tensor, norb: int, na: int, nb: int) -> torch.Tensor | None:
+    n = _lib.check(
+        _lib.lib().ffsim_orbital_rotation_work_elements(dtype_tag(t), norb, na, nb)
+    )
+    if n == 0:
+        return None
+    return torch.empty(n, dtype=t.dtype, device=t.device)

@@ -1,0 +1,59 @@
+"""scipy LinearOperator whose matvec runs on the GPU.
+
+numpy vectors go through scipy's usual path (copied to HBM and back); torch
+tensors on the GPU are handled resident by ``matvec``/``@`` without leaving
+the device.  ``matvec == rmatvec`` as in ffsim (the operators are Hermitian).
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Callable
+
+import numpy as np
+import torch
+from scipy.sparse.linalg import LinearOperator
+
+from ffsim_amd import _device
+
+
+def _dim(norb, nelec):
+    if isinstance(nelec, int):
+        return math.comb(norb, nelec)
+    return math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+
+
+class DeviceLinearOperator(LinearOperator):
+    def __init__(self, norb: int, nelec, matvec_device: Callable[[torch.Tensor], torch.Tensor]):
+        n = _dim(norb, nelec)
+        super().__init__(dtype=np.complex128, shape=(n, n))
+        self.norb = norb
+        self.nelec = nelec
+        self.matvec_device = matvec_device
+
+    def _matvec(self, x):
+        x = np.asarray(x)
+        shape = x.shape
+        st = _device.to_device(x.reshape(-1), copy=False)
+        out = self.matvec_device(st.tensor)
+        return st.restore(out).reshape(shape)
+
+    _rmatvec = _matvec
+
+    def _adjoint(self):
+        return self
+
+    def matvec(self, x):
+        if isinstance(x, torch.Tensor) and x.device.type == "cuda":
+            return self.matvec_device(x.reshape(-1))
+        return super().matvec(x)
+
+    def __matmul__(self, x):
+        if isinstance(x, torch.Tensor) and x.device.type == "cuda":
+            return self.matvec_device(x.reshape(-1))
+        return super().__matmul__(x)
+
+    def dot(self, x):
+        if isinstance(x, torch.Tensor) and x.device.type == "cuda":
+            return self.matvec_device(x.reshape(-1))
+        return super().dot(x)

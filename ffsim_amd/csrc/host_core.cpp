@@ -1,0 +1,341 @@
+// Host-side algorithms of the ffsim_amd library (see host_core.hpp).
+#include "host_core.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace ffsim_amd {
+
+namespace {
+struct BinomTable {
+  uint64_t v[65][65];
+  BinomTable() {
+    for (int n = 0; n <= 64; ++n) {
+      for (int k = 0; k <= 64; ++k) v[n][k] = 0;
+      v[n][0] = 1;
+      for (int k = 1; k <= n; ++k) v[n][k] = v[n - 1][k - 1] + (k <= n - 1 ? v[n - 1][k] : 0);
+    }
+  }
+};
+const BinomTable& binoms() {
+  static const BinomTable t;
+  return t;
+}
+
+// next bitmask with the same popcount (Gosper)
+inline uint64_t next_same_popcount(uint64_t x) {
+  uint64_t c = x & (~x + 1);
+  uint64_t r = x + c;
+  return (((r ^ x) >> 2) / c) | r;
+}
+}  // namespace
+
+uint64_t binom(int n, int k) {
+  if (n < 0 || k < 0 || n > 64 || k > 64 || k > n) return 0;
+  return binoms().v[n][k];
+}
+
+std::vector<uint64_t> make_strings(int norb, int nocc) {
+  std::vector<uint64_t> out;
+  if (norb < 0 || nocc < 0 || nocc > norb || norb > 63) return out;
+  uint64_t count = binom(norb, nocc);
+  out.reserve(count);
+  if (nocc == 0) {
+    out.push_back(0);
+    return out;
+  }
+  uint64_t s = (nocc == 64) ? ~0ull : ((1ull << nocc) - 1);
+  for (uint64_t a = 0; a < count; ++a) {
+    out.push_back(s);
+    if (a + 1 < count) s = next_same_popcount(s);
+  }
+  return out;
+}
+
+uint64_t string_rank(uint64_t s) {
+  uint64_t r = 0;
+  int t = 0;
+  while (s) {
+    int c = __builtin_ctzll(s);
+    r += binom(c, t + 1);
+    ++t;
+    s &= s - 1;
+  }
+  return r;
+}
+
+WindowLayout make_window_layout(int norb, int nocc, int lo, int w) {
+  if (lo < 0 || w < 1 || lo + w > norb) throw std::invalid_argument("bad window");
+  WindowLayout L;
+  L.lo = lo;
+  L.w = w;
+  L.contiguous = (lo == 0);
+  const uint64_t win_mask = (w == 64 ? ~0ull : ((1ull << w) - 1)) << lo;
+  const uint64_t all_mask = (norb == 64 ? ~0ull : ((1ull << norb) - 1));
+  const uint64_t out_mask = all_mask & ~win_mask;
+  const int n_out = norb - w;
+  struct Tmp {
+    uint64_t fixed;
+    int kp;
+  };
+  std::vector<Tmp> act, triv;
+  // enumerate subsets of the outside orbitals in ascending order of the fixed pattern
+  std::vector<int> out_pos;
+  for (int b = 0; b < norb; ++b)
+    if ((out_mask >> b) & 1) out_pos.push_back(b);
+  for (uint64_t m = 0; m < (1ull << n_out); ++m) {
+    uint64_t fixed = 0;
+    for (int t = 0; t < n_out; ++t)
+      if ((m >> t) & 1) fixed |= 1ull << out_pos[t];
+    int e = __builtin_popcountll(fixed);
+    int kp = nocc - e;
+    if (kp < 0 || kp > w) continue;
+    if (kp > 0 && kp < w)
+      act.push_back({fixed, kp});
+    else
+      triv.push_back({fixed, kp});
+  }
+  // largest fibers first so that the heaviest tiles start early
+  std::stable_sort(act.begin(), act.end(),
+                   [&](const Tmp& a, const Tmp& b) { return binom(w, a.kp) > binom(w, b.kp); });
+  auto emit = [&](const Tmp& t) {
+    Fiber f;
+    f.row_off = (int)L.row_table.size();
+    f.kp = t.kp;
+    f.n_rows = (int)binom(w, t.kp);
+    f.pad = 0;
+    std::vector<uint64_t> locals = make_strings(w, t.kp);
+    for (uint64_t loc : locals) L.row_table.push_back((int32_t)string_rank(t.fixed | (loc << lo)));
+    L.max_rows = std::max(L.max_rows, f.n_rows);
+    L.fibers.push_back(f);
+  };
+  for (auto& t : act) {
+    emit(t);
+    L.max_rows_active = std::max(L.max_rows_active, (int)binom(w, t.kp));
+  }
+  L.n_active = (int)act.size();
+  for (auto& t : triv) emit(t);
+  return L;
+}
+
+PairTables make_pair_tables(int w) {
+  PairTables T;
+  T.w = w;
+  int wm1 = std::max(w - 1, 1);
+  T.index.assign((size_t)(w + 1) * wm1 * 2, 0);
+  for (int kp = 0; kp <= w; ++kp) {
+    std::vector<uint64_t> locals = make_strings(w, kp);
+    for (int p = 0; p + 1 < w; ++p) {
+      int off = (int)T.pairs.size();
+      for (size_t r = 0; r < locals.size(); ++r) {
+        uint64_t s = locals[r];
+        if (((s >> p) & 1) && !((s >> (p + 1)) & 1)) {
+          uint64_t y = s ^ (3ull << p);
+          uint32_t ry = (uint32_t)string_rank(y);
+          T.pairs.push_back((uint32_t)r | (ry << 16));
+        }
+      }
+      T.index[((size_t)kp * wm1 + p) * 2 + 0] = off;
+      T.index[((size_t)kp * wm1 + p) * 2 + 1] = (int)T.pairs.size() - off;
+    }
+  }
+  return T;
+}
+
+// ------------------------------------------------------------------ Givens
+namespace {
+// Rotation that zeroes b against a: returns (c, s) with c real such that
+// c*b - conj(s)*a == 0 for the column/row updates below (BLAS zrotg
+// convention with thresholds, as the decomposition contract requires).
+void zrotg_tol(cdouble a, cdouble b, double tol, double& c, cdouble& s) {
+  double aa = std::abs(a), ab = std::abs(b);
+  if (ab <= tol) {
+    c = 1.0;
+    s = 0.0;
+    return;
+  }
+  if (aa <= tol) {
+    c = 0.0;
+    s = 1.0;
+    return;
+  }
+  double r = std::hypot(aa, ab);
+  c = std::min(1.0, std::max(-1.0, aa / r));
+  s = (a / aa) * std::conj(b) / r;
+}
+}  // namespace
+
+GivensDecomposition givens_decomposition(const cdouble* mat, int n, double tol) {
+  GivensDecomposition out;
+  std::vector<cdouble> M(mat, mat + (size_t)n * n);
+  auto at = [&](int r, int c) -> cdouble& { return M[(size_t)r * n + c]; };
+  // column update on (x, y): x <- c x + s y, y <- c y - conj(s) x  (M <- M G^T)
+  auto rot_cols = [&](int x, int y, double c, cdouble s) {
+    for (int r = 0; r < n; ++r) {
+      cdouble xo = at(r, x), yo = at(r, y);
+      at(r, x) = c * xo + s * yo;
+      at(r, y) = c * yo - std::conj(s) * xo;
+    }
+  };
+  // row update on (x, y) with the same 2x2 (M <- G M)
+  auto rot_rows = [&](int x, int y, double c, cdouble s) {
+    for (int k = 0; k < n; ++k) {
+      cdouble xo = at(x, k), yo = at(y, k);
+      at(x, k) = c * xo + s * yo;
+      at(y, k) = c * yo - std::conj(s) * xo;
+    }
+  };
+  std::vector<GivensRotation> right, left;
+  // Clements et al., Optica 3, 1460 (2016): alternate nulling anti-diagonals
+  // from the right (column rotations) and from the left (row rotations).
+  for (int d = 0; d + 1 < n; ++d) {
+    if (d % 2 == 0) {
+      for (int t = 0; t <= d; ++t) {
+        int col = d - t, row = n - 1 - t;
+        if (std::abs(at(row, col)) > tol) {
+          double c;
+          cdouble s;
+          zrotg_tol(at(row, col + 1), at(row, col), tol, c, s);
+          right.push_back({c, s, col + 1, col});
+          rot_cols(col + 1, col, c, s);
+        }
+      }
+    } else {
+      for (int t = 0; t <= d; ++t) {
+        int row = n - 1 - d + t, col = t;
+        if (std::abs(at(row, col)) > tol) {
+          double c;
+          cdouble s;
+          zrotg_tol(at(row - 1, col), at(row, col), tol, c, s);
+          left.push_back({c, s, row - 1, row});
+          rot_rows(row - 1, row, c, s);
+        }
+      }
+    }
+  }
+  // Now L_K ... L_1 U G_1^T ... G_J^T = diag(d), i.e.
+  // U = L_1^H ... L_K^H diag(d) G_J^* ... G_1^*.  Move each L^H through the
+  // diagonal: L^H diag(d) = diag(d) G'^* with G' = (c, -conj(s) d_i conj(d_j)).
+  std::vector<cdouble> d(n);
+  for (int k = 0; k < n; ++k) d[k] = at(k, k);
+  out.rotations = right;
+  for (auto it = left.rbegin(); it != left.rend(); ++it) {
+    const GivensRotation& g = *it;
+    cdouble sp = -std::conj(g.s) * d[g.i] * std::conj(d[g.j]);
+    out.rotations.push_back({g.c, sp, g.i, g.j});
+  }
+  out.diag = d;
+  return out;
+}
+
+KStep kernel_step_from_rotation(const GivensRotation& g) {
+  KStep k{};
+  cdouble sigma;
+  if (g.j == g.i + 1) {
+    k.p = g.i;
+    sigma = std::conj(g.s);
+  } else if (g.i == g.j + 1) {
+    k.p = g.j;
+    sigma = -g.s;
+  } else {
+    throw std::invalid_argument("Target orbitals must be adjacent.");
+  }
+  k.c = g.c;
+  k.sr = sigma.real();
+  k.si = sigma.imag();
+  return k;
+}
+
+namespace {
+std::vector<PlannedPass> plan_from(const std::vector<KStep>& steps, int norb, int w,
+                                   const std::vector<int>& lows, int start) {
+  std::vector<PlannedPass> passes;
+  std::vector<char> done(steps.size(), 0);
+  size_t n_done = 0;
+  int m = (int)lows.size();
+  // zigzag over the windows: start, start+1, ..., m-1, m-2, ..., 0, 1, ...
+  std::vector<int> order;
+  if (m == 1) {
+    order = {0};
+  } else {
+    for (int k = 0; k < m; ++k) order.push_back(k);
+    for (int k = m - 2; k >= 1; --k) order.push_back(k);
+  }
+  int pos = 0;
+  for (int k = 0; k < (int)order.size(); ++k)
+    if (order[k] == start) pos = k;
+  int idle = 0;
+  while (n_done < steps.size()) {
+    int lo = lows[order[pos % order.size()]];
+    ++pos;
+    PlannedPass P;
+    P.lo = lo;
+    std::vector<char> blocked(norb + 1, 0);
+    for (size_t r = 0; r < steps.size(); ++r) {
+      if (done[r]) continue;
+      int p = steps[r].p;
+      bool inside = p >= lo && p + 1 < lo + w;
+      if (inside && !blocked[p] && !blocked[p + 1]) {
+        KStep k = steps[r];
+        k.p = p - lo;
+        P.steps.push_back(k);
+        done[r] = 1;
+        ++n_done;
+      } else {
+        blocked[p] = blocked[p + 1] = 1;
+      }
+    }
+    if (!P.steps.empty()) {
+      passes.push_back(std::move(P));
+      idle = 0;
+    } else if (++idle > 2 * m + 2) {
+      throw std::runtime_error("pass planner made no progress");
+    }
+  }
+  return passes;
+}
+}  // namespace
+
+std::vector<PlannedPass> plan_passes(const std::vector<KStep>& steps, int norb, int w,
+                                     bool end_low, int first_window) {
+  if (w >= norb) {
+    std::vector<PlannedPass> one(1);
+    one[0].lo = 0;
+    one[0].steps = steps;
+    one[0].last = true;
+    return one;
+  }
+  // window starts: evenly spaced from 0 to norb - w with overlapping windows
+  int span = norb - w;
+  int n_win = 2;
+  while ((span + n_win - 2) / (n_win - 1) > w - 1) ++n_win;
+  std::vector<int> lows;
+  for (int k = 0; k < n_win; ++k) lows.push_back((int)((long)span * k / (n_win - 1)));
+  std::vector<PlannedPass> best;
+  long best_cost = -1;
+  for (int start : {0, n_win - 1}) {
+    if (first_window >= 0 && start != (first_window == 0 ? 0 : n_win - 1)) continue;
+    std::vector<PlannedPass> p = plan_from(steps, norb, w, lows, start);
+    long cost = (long)p.size();
+    if (end_low && (p.empty() || p.back().lo != 0)) cost += 1;
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = std::move(p);
+    }
+  }
+  if (end_low && (best.empty() || best.back().lo != 0)) {
+    PlannedPass P;
+    P.lo = 0;
+    best.push_back(P);
+  }
+  if (best.empty()) {
+    PlannedPass P;
+    P.lo = 0;
+    best.push_back(P);
+  }
+  best.back().last = true;
+  return best;
+}
+
+}  // namespace ffsim_amd

@@ -1,0 +1,109 @@
+// Host-side algorithms of the ffsim_amd library: determinant-string tables,
+// window fibers, the Clements Givens decomposition and the fused-pass planner.
+// Everything here is plain FP64 / integer C++; the HIP side only consumes the
+// tables and programs these routines produce.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+namespace ffsim_amd {
+
+using cdouble = std::complex<double>;
+
+// ---------------------------------------------------------------- binomials
+// C(n, k) for 0 <= n, k <= 64 (0 when k > n).
+uint64_t binom(int n, int k);
+
+// ------------------------------------------------------------ string tables
+// All k-subsets of norb orbitals as bitmasks in ascending integer order.  The
+// position of a string in this list is its address (the colex rank), the
+// ordering ffsim uses for CI strings (python/ffsim/_cistring.py:21-24 via
+// pyscf cistring.make_strings; tests/python/states/bitstring_test.py:24-97).
+std::vector<uint64_t> make_strings(int norb, int nocc);
+// Colex rank of a bitmask: sum over set bits t-th (from 0) at position c_t of
+// C(c_t, t + 1).
+uint64_t string_rank(uint64_t s);
+
+// Fibers of a window of w consecutive orbitals [lo, lo + w).  A fiber is the
+// set of strings that agree on every orbital outside the window; its local
+// index is the colex rank of the window bits.  Any Givens rotation on an
+// adjacent pair (p, p + 1) inside the window maps each fiber onto itself, and
+// no orbital lies between p and p + 1, so no fermionic sign appears.
+struct Fiber {
+  int row_off;  // offset of this fiber in the window's row table
+  int n_rows;   // C(w, kp)
+  int kp;       // electrons inside the window
+  int pad;
+};
+
+struct WindowLayout {
+  int lo = 0, w = 0;
+  std::vector<Fiber> fibers;      // active fibers (0 < kp < w) first, then trivial ones
+  int n_active = 0;
+  std::vector<int32_t> row_table; // local -> global address, fibers concatenated
+  int max_rows = 0;               // largest fiber over all fibers
+  int max_rows_active = 0;
+  bool contiguous = false;        // every fiber is a contiguous address range
+};
+WindowLayout make_window_layout(int norb, int nocc, int lo, int w);
+
+// Local pair tables for a window of width w: for each kp in [0, w] and each
+// adjacent local pair (p, p + 1), the list of (x, y) local ranks with x having
+// p occupied / p + 1 empty and y = x with the electron moved to p + 1.
+// Entries are packed as x | (y << 16).
+struct PairTables {
+  int w = 0;
+  std::vector<uint32_t> pairs;
+  std::vector<int32_t> index;  // [(kp * (w - 1) + p) * 2 + {0: offset, 1: count}]
+};
+PairTables make_pair_tables(int w);
+
+// ------------------------------------------------------ Givens decomposition
+// Exact Clements decomposition U = D G_L^* ... G_1^* with each G acting on
+// adjacent orbitals (i, j) as [[c, s], [-conj(s), c]] (c real), matching the
+// output contract of ffsim.linalg.givens_decomposition
+// (python/ffsim/linalg/givens.py:59-156, src/linalg/givens.rs:69-149).
+struct GivensRotation {
+  double c;
+  cdouble s;
+  int i, j;
+};
+struct GivensDecomposition {
+  std::vector<GivensRotation> rotations;  // in application order G_1, G_2, ...
+  std::vector<cdouble> diag;              // D
+};
+// mat is row-major n x n.
+GivensDecomposition givens_decomposition(const cdouble* mat, int n, double tol);
+
+// ----------------------------------------------------- fused-pass planning
+// A rotation in kernel form: on the local pair x = (p occupied, p+1 empty),
+// y = (p empty, p+1 occupied) it applies x' = c x + sigma y,
+// y' = c y - conj(sigma) x.
+struct KStep {
+  int p;
+  int pad;
+  double c, sr, si;
+};
+static_assert(sizeof(KStep) == 32, "KStep layout is shared with the device");
+
+// Converts a decomposition rotation (c, s, i, j) into the kernel step that
+// reproduces ffsim's application (python/ffsim/gates/orbital_rotation.py:132-135
+// applies (c, conj(s)) to the rows with i occupied / j occupied).
+KStep kernel_step_from_rotation(const GivensRotation& g);
+
+struct PlannedPass {
+  int lo;                    // window start (orbital)
+  std::vector<KStep> steps;  // window-local p
+  bool last = false;         // final pass of the network
+};
+// Greedy window schedule: alternate between the windows [0, w) and
+// [norb - w, norb), each pass applying every rotation whose pair lies inside
+// the window and whose predecessors (earlier rotations sharing an orbital)
+// are already applied.  If end_low is set, the last pass is forced onto the
+// window that starts at orbital 0 (its fibers are contiguous address ranges).
+std::vector<PlannedPass> plan_passes(const std::vector<KStep>& global_steps, int norb, int w,
+                                     bool end_low, int first_window);
+
+}  // namespace ffsim_amd

@@ -1,0 +1,81 @@
+"""Double-factorized Hamiltonian linear operator
+(python/ffsim/hamiltonians/double_factorized_hamiltonian.py)."""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import scipy.linalg
+import torch
+
+from ffsim_amd import _device, _lib
+from ffsim_amd._linop import DeviceLinearOperator
+from ffsim_amd.contract.diag_coulomb import contract_diag_coulomb_into
+from ffsim_amd.contract.num_op_sum import contract_num_op_sum_into
+from ffsim_amd.gates.orbital_rotation import orbital_rotation_in_place
+
+
+def axpby_(out: torch.Tensor, x: torch.Tensor, y: torch.Tensor | None, alpha, beta=0.0):
+    """out = alpha x + beta y on the device (out may alias x or y)."""
+    a = complex(alpha)
+    b = complex(beta)
+    _lib.call(
+        "ffsim_axpby",
+        _device.ptr(out),
+        _device.ptr(x),
+        _device.ptr(y),
+        _device.dtype_tag(out),
+        out.numel(),
+        a.real,
+        a.imag,
+        b.real,
+        b.imag,
+        _device.stream_ptr(),
+    )
+    return out
+
+
+@dataclasses.dataclass(frozen=True)
+class DoubleFactorizedHamiltonian:
+    r"""H = sum_{pq,sigma} kappa_pq a^dag a + 1/2 sum_t sum J^t_ij n^t_i n^t_j + constant."""
+
+    one_body_tensor: np.ndarray
+    diag_coulomb_mats: np.ndarray
+    orbital_rotations: np.ndarray
+    constant: float = 0.0
+    z_representation: bool = False
+
+    @property
+    def norb(self) -> int:
+        return self.one_body_tensor.shape[0]
+
+    def _linear_operator_(self, norb: int, nelec) -> DeviceLinearOperator:
+        """Same operator as ffsim's (double_factorized_hamiltonian.py:244-275): the
+        one-body term as a rotated number-operator sum, plus one rotated
+        diagonal Coulomb contraction per factor, evaluated on the GPU."""
+        assert isinstance(nelec, tuple)
+        eigs, vecs = scipy.linalg.eigh(self.one_body_tensor)
+        mats = [np.asarray(m, dtype=float) for m in self.diag_coulomb_mats]
+        rots = [np.asarray(u, dtype=np.complex128) for u in self.orbital_rotations]
+        constant = self.constant
+        z_rep = self.z_representation
+
+        def matvec_device(v: torch.Tensor) -> torch.Tensor:
+            result = torch.empty_like(v)
+            axpby_(result, v, None, constant)
+            t = torch.empty_like(v)
+            t.copy_(v)
+            orbital_rotation_in_place(t, vecs.T.conj(), norb, nelec)
+            contract_num_op_sum_into(t, t, eigs, norb, nelec)
+            orbital_rotation_in_place(t, vecs, norb, nelec)
+            axpby_(result, result, t, 1.0, 1.0)
+            for mat, rot in zip(mats, rots):
+                t.copy_(v)
+                orbital_rotation_in_place(t, rot.T.conj(), norb, nelec)
+                contract_diag_coulomb_into(t, t, mat, norb, nelec, z_rep)
+                orbital_rotation_in_place(t, rot, norb, nelec)
+                axpby_(result, result, t, 1.0, 1.0)
+            return result
+
+        return DeviceLinearOperator(norb, nelec, matvec_device)

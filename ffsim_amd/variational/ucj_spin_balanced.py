@@ -1,0 +1,220 @@
+"""Spin-balanced (L)UCJ operator (python/ffsim/variational/ucj_spin_balanced.py).
+
+The dataclass mirrors ffsim's: the same fields, validation, parameter
+layout and ``_apply_unitary_`` semantics.  ``_apply_unitary_`` runs the whole
+ansatz as one C-ABI call on the GPU (fused window passes for every orbital
+rotation, one elementwise pass per diagonal Coulomb evolution).
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import InitVar, dataclass
+
+import numpy as np
+import torch
+
+from ffsim_amd import _device, _lib
+from ffsim_amd import linalg as _linalg
+
+
+def _validate_interaction_pairs(pairs, ordered: bool) -> None:
+    """python/ffsim/variational/util.py validate_interaction_pairs."""
+    if pairs is None:
+        return
+    if len(set(pairs)) != len(pairs):
+        raise ValueError(f"Duplicate interaction pairs encountered: {pairs}.")
+    if not ordered:
+        for i, j in pairs:
+            if i > j:
+                raise ValueError(
+                    "When specifying spinless, alpha-alpha or beta-beta interaction pairs, "
+                    f"you must provide only upper triangular pairs. Got {(i, j)}, which is a "
+                    "lower triangular pair."
+                )
+
+
+@dataclass(frozen=True)
+class UCJOpSpinBalanced:
+    r"""A spin-balanced unitary cluster Jastrow operator
+    :math:`\prod_k \mathcal{U}_k e^{i \mathcal{J}_k} \mathcal{U}_k^\dagger`."""
+
+    diag_coulomb_mats: np.ndarray  # (n_reps, 2, norb, norb)
+    orbital_rotations: np.ndarray  # (n_reps, norb, norb)
+    final_orbital_rotation: np.ndarray | None = None
+    validate: InitVar[bool] = True
+    rtol: InitVar[float] = 1e-5
+    atol: InitVar[float] = 1e-8
+
+    def __post_init__(self, validate: bool, rtol: float, atol: float):
+        if not validate:
+            return
+        if self.diag_coulomb_mats.ndim != 4 or self.diag_coulomb_mats.shape[1] != 2:
+            raise ValueError(
+                "diag_coulomb_mats should have shape (n_reps, 2, norb, norb). "
+                f"Got shape {self.diag_coulomb_mats.shape}."
+            )
+        if self.orbital_rotations.ndim != 3:
+            raise ValueError(
+                "orbital_rotations should have shape (n_reps, norb, norb). "
+                f"Got shape {self.orbital_rotations.shape}."
+            )
+        if self.final_orbital_rotation is not None and self.final_orbital_rotation.ndim != 2:
+            raise ValueError(
+                "final_orbital_rotation should have shape (norb, norb). "
+                f"Got shape {self.final_orbital_rotation.shape}."
+            )
+        if self.diag_coulomb_mats.shape[0] != self.orbital_rotations.shape[0]:
+            raise ValueError(
+                "diag_coulomb_mats and orbital_rotations should have the same first dimension. "
+                f"Got {self.diag_coulomb_mats.shape[0]} and {self.orbital_rotations.shape[0]}."
+            )
+        if not all(
+            _linalg.is_real_symmetric(m[0], rtol=rtol, atol=atol)
+            and _linalg.is_real_symmetric(m[1], rtol=rtol, atol=atol)
+            for m in self.diag_coulomb_mats
+        ):
+            raise ValueError("Diagonal Coulomb matrices were not all real symmetric.")
+        if not all(_linalg.is_unitary(u, rtol=rtol, atol=atol) for u in self.orbital_rotations):
+            raise ValueError("Orbital rotations were not all unitary.")
+        if self.final_orbital_rotation is not None and not _linalg.is_unitary(
+            self.final_orbital_rotation, rtol=rtol, atol=atol
+        ):
+            raise ValueError("Final orbital rotation was not unitary.")
+
+    @property
+    def norb(self) -> int:
+        return self.diag_coulomb_mats.shape[-1]
+
+    @property
+    def n_reps(self) -> int:
+        return self.diag_coulomb_mats.shape[0]
+
+    @staticmethod
+    def n_params(norb: int, n_reps: int, *, interaction_pairs=None,
+                 with_final_orbital_rotation: bool = False) -> int:
+        """Number of parameters (python/ffsim/variational/ucj_spin_balanced.py:143-195)."""
+        pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
+        _validate_interaction_pairs(pairs_aa, ordered=False)
+        _validate_interaction_pairs(pairs_ab, ordered=False)
+        n_triu = norb * (norb + 1) // 2
+        n_aa = n_triu if pairs_aa is None else len(pairs_aa)
+        n_ab = n_triu if pairs_ab is None else len(pairs_ab)
+        return n_reps * (n_aa + n_ab + norb**2) + with_final_orbital_rotation * norb**2
+
+    @staticmethod
+    def from_parameters(params: np.ndarray, *, norb: int, n_reps: int, interaction_pairs=None,
+                        with_final_orbital_rotation: bool = False) -> "UCJOpSpinBalanced":
+        """Operator from a real parameter vector (ucj_spin_balanced.py:197-293)."""
+        n_params = UCJOpSpinBalanced.n_params(
+            norb, n_reps, interaction_pairs=interaction_pairs,
+            with_final_orbital_rotation=with_final_orbital_rotation,
+        )
+        if len(params) != n_params:
+            raise ValueError(
+                "The number of parameters passed did not match the number expected "
+                "based on the function inputs. "
+                f"Expected {n_params} but got {len(params)}."
+            )
+        pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
+        triu = list(itertools.combinations_with_replacement(range(norb), 2))
+        pairs_aa = triu if pairs_aa is None else pairs_aa
+        pairs_ab = triu if pairs_ab is None else pairs_ab
+        diag_coulomb_mats = np.zeros((n_reps, 2, norb, norb))
+        orbital_rotations = np.zeros((n_reps, norb, norb), dtype=complex)
+        index = 0
+        for rep in range(n_reps):
+            orbital_rotations[rep] = _linalg.unitary_from_parameters(
+                params[index : index + norb**2], norb
+            )
+            index += norb**2
+            for kind, pairs in enumerate((pairs_aa, pairs_ab)):
+                if pairs:
+                    rows, cols = zip(*pairs)
+                    vals = params[index : index + len(pairs)]
+                    diag_coulomb_mats[rep, kind][cols, rows] = vals
+                    diag_coulomb_mats[rep, kind][rows, cols] = vals
+                    index += len(pairs)
+        final = None
+        if with_final_orbital_rotation:
+            final = _linalg.unitary_from_parameters(params[index:], norb)
+        return UCJOpSpinBalanced(
+            diag_coulomb_mats=diag_coulomb_mats,
+            orbital_rotations=orbital_rotations,
+            final_orbital_rotation=final,
+        )
+
+    def to_parameters(self, *, interaction_pairs=None) -> np.ndarray:
+        """Real parameter vector (ucj_spin_balanced.py:296-373)."""
+        n_reps, _, norb, _ = self.diag_coulomb_mats.shape
+        n_params = UCJOpSpinBalanced.n_params(
+            norb, n_reps, interaction_pairs=interaction_pairs,
+            with_final_orbital_rotation=self.final_orbital_rotation is not None,
+        )
+        pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
+        triu = list(itertools.combinations_with_replacement(range(norb), 2))
+        pairs_aa = triu if pairs_aa is None else pairs_aa
+        pairs_ab = triu if pairs_ab is None else pairs_ab
+        params = np.zeros(n_params)
+        index = 0
+        for rot, mats in zip(self.orbital_rotations, self.diag_coulomb_mats):
+            params[index : index + norb**2] = _linalg.unitary_to_parameters(rot)
+            index += norb**2
+            for pairs, m in zip((pairs_aa, pairs_ab), mats):
+                if pairs:
+                    params[index : index + len(pairs)] = m[tuple(zip(*pairs))]
+                    index += len(pairs)
+        if self.final_orbital_rotation is not None:
+            params[index:] = _linalg.unitary_to_parameters(self.final_orbital_rotation)
+        return params
+
+    def apply_in_place(self, t: torch.Tensor, norb: int, nelec) -> torch.Tensor:
+        """Apply to a resident 1-D device state in place."""
+        na, nb = nelec
+        _device.check_size(t, norb, nelec)
+        dcm = np.ascontiguousarray(self.diag_coulomb_mats, dtype=float)
+        rots = np.ascontiguousarray(self.orbital_rotations, dtype=np.complex128)
+        if dcm.shape[-1] != norb or rots.shape[-1] != norb:
+            raise ValueError(f"operator has norb={self.norb}, expected {norb}")
+        final = (
+            np.ascontiguousarray(self.final_orbital_rotation, dtype=np.complex128)
+            if self.final_orbital_rotation is not None
+            else None
+        )
+        work = _device.work_buffer(t, norb, na, nb)
+        _lib.call(
+            "ffsim_apply_ucj_spin_balanced",
+            _device.ptr(t),
+            _device.ptr(work),
+            _device.dtype_tag(t),
+            norb,
+            na,
+            nb,
+            self.n_reps,
+            _device.host_ptr(dcm),
+            _device.host_ptr(rots),
+            _device.host_ptr(final),
+            _device.stream_ptr(),
+        )
+        return t
+
+    def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
+        if isinstance(nelec, int):
+            return NotImplemented
+        st = _device.to_device(vec, copy)
+        self.apply_in_place(st.tensor, norb, nelec)
+        return st.restore(st.tensor)
+
+    def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
+        if isinstance(other, UCJOpSpinBalanced):
+            if not np.allclose(self.diag_coulomb_mats, other.diag_coulomb_mats, rtol=rtol, atol=atol):
+                return False
+            if not np.allclose(self.orbital_rotations, other.orbital_rotations, rtol=rtol, atol=atol):
+                return False
+            if (self.final_orbital_rotation is None) != (other.final_orbital_rotation is None):
+                return False
+            if self.final_orbital_rotation is not None:
+                return np.allclose(self.final_orbital_rotation, other.final_orbital_rotation,
+                                   rtol=rtol, atol=atol)
+            return True
+        return NotImplemented

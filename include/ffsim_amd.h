@@ -1,0 +1,145 @@
+/*
+ * ffsim_amd: C ABI of the MI355X (gfx950) gate-application library.
+ *
+ * Every entry point returns 0 on success and a negative status on failure;
+ * ffsim_amd_last_error() then describes the failure (thread-local string).
+ *
+ * State vectors are device pointers to row-major dim_a x dim_b complex
+ * matrices (address = addr_a * dim_b + addr_b, python/ffsim/states/bitstring.py:246),
+ * interleaved (re, im), in FP64 (dtype FFSIM_C128) or FP32 (dtype FFSIM_C64).
+ * Matrices describing gates (unitaries, Coulomb matrices, coefficients) are
+ * HOST pointers to FP64 data; complex matrices are interleaved complex128,
+ * row-major, exactly the memory of a C-contiguous numpy complex128 array.
+ * `stream` is a hipStream_t (NULL = the default stream); all device work of a
+ * call is enqueued on it and the call returns without synchronizing.
+ * Scratch state buffers (`work`) are owned by the caller and must hold
+ * dim_a * dim_b elements of the state dtype.
+ */
+#ifndef FFSIM_AMD_H
+#define FFSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFSIM_C128 0
+#define FFSIM_C64 1
+
+#define FFSIM_OK 0
+#define FFSIM_ERR_VALUE (-1)   /* bad argument (python ValueError)            */
+#define FFSIM_ERR_ASSERT (-2)  /* violated precondition (python AssertionError) */
+#define FFSIM_ERR_HIP (-3)     /* HIP runtime failure                          */
+#define FFSIM_ERR_NOTIMPL (-4) /* unsupported combination (NotImplementedError) */
+
+/* Library status. */
+const char* ffsim_amd_last_error(void);
+int ffsim_amd_version(void);
+/* Selects and initialises `device` (tables, program ring).  Optional: every
+ * device entry point initialises the current device on first use. */
+int ffsim_amd_init(int device);
+
+/* Kernel timing (benchmark support; no counterpart in the reference).
+ * Between begin and end every fiber (Givens) kernel and diagonal kernel launch
+ * is bracketed by HIP events on the stream it runs on.  end synchronizes and
+ * writes 6 doubles: {fiber ms, fiber launches, fiber algorithmic bytes,
+ * diagonal ms, diagonal launches, diagonal algorithmic bytes}. */
+int ffsim_amd_profile_begin(void);
+int ffsim_amd_profile_end(double* out);
+
+/* ---- host-side tables (bit-exact with ffsim's cached pyscf tables) ----- */
+
+/* Replaces ffsim._cistring.make_strings (python/ffsim/_cistring.py:21-24):
+ * the C(norb, nocc) occupation bitmasks in ascending order. */
+int ffsim_make_strings(int norb, int nocc, int64_t* out);
+/* Replaces ffsim._cistring.gen_occslst (python/ffsim/_cistring.py:27-30):
+ * out is C(norb, nocc) x nocc, occupied orbitals ascending per string. */
+int ffsim_gen_occslst(int norb, int nocc, uint64_t* out);
+/* Replaces gates.orbital_rotation._zero_one_subspace_indices
+ * (python/ffsim/gates/orbital_rotation.py:202-212): 2 * C(norb-2, nocc-1)
+ * addresses, the strings with i occupied / j empty followed by their partners
+ * with j occupied / i empty.  Returns the number of entries written. */
+int ffsim_zero_one_subspace_indices(int norb, int nocc, int i, int j, uint64_t* out);
+
+/* Replaces ffsim._lib.givens_decomposition (src/linalg/givens.rs:69-149,
+ * python/ffsim/_lib.pyi:42-44).  mat: n x n complex128.  Writes up to
+ * n(n-1)/2 rotations (c[k], s[2k..2k+1], i[k], j[k]) with U = D G_L^* ... G_1^*
+ * and the n diagonal phases (2n doubles).  *n_rot receives L. */
+int ffsim_givens_decomposition(const double* mat, int n, double tol, double* c, double* s,
+                               int32_t* i, int32_t* j, int32_t* n_rot, double* diag);
+
+/* ---- device gate application ------------------------------------------ */
+
+/* Number of state elements the `work` buffer of ffsim_apply_orbital_rotation
+ * and ffsim_apply_ucj_spin_balanced needs for this shape (0: none, NULL may
+ * be passed).  Negative on error.  No reference counterpart: it sizes the
+ * transposition buffer that replaces the reference's transpose copies
+ * (python/ffsim/gates/orbital_rotation.py:143, :152). */
+int64_t ffsim_orbital_rotation_work_elements(int dtype, int norb, int n_alpha, int n_beta);
+
+/* Replaces the spinful path of ffsim.apply_orbital_rotation
+ * (python/ffsim/gates/orbital_rotation.py:117-154 -> _lib.apply_givens_rotation_in_place,
+ * src/gates/orbital_rotation.rs:19-104, and _lib.apply_phase_shift_in_place,
+ * src/gates/phase_shift.rs:16-30).  mat_a / mat_b: norb x norb complex128 or
+ * NULL (no rotation of that spin sector).  In place on vec; work may be NULL
+ * when norb is small enough that a whole string vector fits on chip. */
+int ffsim_apply_orbital_rotation(void* vec, void* work, int dtype, int norb, int n_alpha,
+                                 int n_beta, const double* mat_a, const double* mat_b,
+                                 void* stream);
+
+/* Replaces _lib.apply_diag_coulomb_evolution_in_place_num_rep / _z_rep
+ * (src/gates/diag_coulomb.rs:19-190).  The matrices are real norb x norb,
+ * already in the form ffsim's _get_mat_exp exponentiates
+ * (python/ffsim/gates/diag_coulomb.py:223-275) but before exponentiation:
+ * number representation -> diagonals of aa/bb halved; Z representation ->
+ * raw matrices (the 1/4 factor is applied here).  NULL = no interaction of
+ * that type.  Applies vec[a,b] *= exp(-i time E(a,b)). */
+int ffsim_apply_diag_coulomb_evolution(void* vec, int dtype, int norb, int n_alpha, int n_beta,
+                                       const double* mat_aa, const double* mat_ab,
+                                       const double* mat_bb, double time, int z_representation,
+                                       void* stream);
+
+/* Replaces _lib.apply_num_op_sum_evolution_in_place on both spin sectors
+ * (src/gates/num_op_sum.rs:18-36, python/ffsim/gates/num_op_sum.py:175-219).
+ * coeffs_a / coeffs_b: norb real coefficients or NULL. */
+int ffsim_apply_num_op_sum_evolution(void* vec, int dtype, int norb, int n_alpha, int n_beta,
+                                     const double* coeffs_a, const double* coeffs_b, double time,
+                                     void* stream);
+
+/* Replaces _lib.contract_diag_coulomb_into_buffer_num_rep / _z_rep
+ * (src/contract/diag_coulomb.rs:19-174): out (=|+=) scale * E(a,b) vec[a,b],
+ * with the matrices prepared as python/ffsim/contract/diag_coulomb.py:102-130
+ * does (number representation: aa/bb diagonals halved).  accumulate = 0
+ * overwrites out, 1 adds into it.  out may alias vec when accumulate = 0. */
+int ffsim_contract_diag_coulomb(const void* vec, void* out, int dtype, int norb, int n_alpha,
+                                int n_beta, const double* mat_aa, const double* mat_ab,
+                                const double* mat_bb, int z_representation, double scale,
+                                int accumulate, void* stream);
+
+/* Replaces _lib.contract_num_op_sum_spin_into_buffer applied to both spin
+ * sectors (src/contract/num_op_sum.rs:18-39, python/ffsim/contract/num_op_sum.py:27-72):
+ * out (=|+=) scale * (sum_{i in a} ca[i] + sum_{j in b} cb[j]) vec[a,b]. */
+int ffsim_contract_num_op_sum(const void* vec, void* out, int dtype, int norb, int n_alpha,
+                              int n_beta, const double* coeffs_a, const double* coeffs_b,
+                              double scale, int accumulate, void* stream);
+
+/* out = alpha * x + beta * y elementwise over n complex entries (device
+ * helper for linear-operator sums; out may alias x or y). */
+int ffsim_axpby(void* out, const void* x, const void* y, int dtype, int64_t n, double alpha_re,
+                double alpha_im, double beta_re, double beta_im, void* stream);
+
+/* Replaces UCJOpSpinBalanced._apply_unitary_
+ * (python/ffsim/variational/ucj_spin_balanced.py:657-696).
+ * diag_coulomb_mats: n_reps x 2 x norb x norb real (aa, ab); orbital_rotations:
+ * n_reps x norb x norb complex128; final_orbital_rotation: NULL or norb x norb. */
+int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, int n_alpha,
+                                  int n_beta, int n_reps, const double* diag_coulomb_mats,
+                                  const double* orbital_rotations,
+                                  const double* final_orbital_rotation, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFSIM_AMD_H */

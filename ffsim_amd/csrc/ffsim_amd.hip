@@ -112,10 +112,13 @@ struct DevWindow {
   int n_fibers = 0, n_active = 0, max_rows = 0, max_rows_active = 0, lo = 0, w = 0;
   long rows_all = 0, rows_active = 0;  // strings covered by all / by the active fibers
 };
-struct DevPairs {
-  uint32_t* pairs = nullptr;
+struct DevGroups {
+  uint32_t* desc = nullptr;
+  uint16_t* members = nullptr;
   int32_t* index = nullptr;
-  int w = 0;
+  std::vector<int32_t> host_index;  // copy of index for LDS sizing on the host
+  int w = 0, w2 = 0, n_lo2 = 0;
+  GroupArgs args() const { return GroupArgs{desc, members, index, n_lo2, w2}; }
 };
 
 // Program ring: per-call gate parameters (Givens steps, phases, diagonal
@@ -139,7 +142,7 @@ struct Context {
   std::mutex mu;
   std::map<std::pair<int, int>, DevStrings> strings;
   std::map<std::tuple<int, int, int, int>, DevWindow> windows;
-  std::map<int, DevPairs> pairs;
+  std::map<int, DevGroups> groups;
   ProgramRing ring;
 };
 
@@ -184,6 +187,7 @@ void prof_stop(long k, hipStream_t stream) {
 
 std::mutex g_ctx_mu;
 std::map<int, std::unique_ptr<Context>> g_ctx;
+int g_n_cu = 256;  // compute units of the current device (persistent grids)
 
 Context& context() {
   int dev = 0;
@@ -197,6 +201,7 @@ Context& context() {
   HIP_CHECK(hipGetDeviceProperties(&prop, dev));
   ctx->lds_max = prop.sharedMemPerBlock;
   ctx->n_cu = prop.multiProcessorCount;
+  g_n_cu = std::max(1, prop.multiProcessorCount);
   for (int k = 0; k < ProgramRing::kSlots; ++k) {
     HIP_CHECK(hipHostMalloc((void**)&ctx->ring.h[k], ProgramRing::kSlotBytes, hipHostMallocDefault));
     HIP_CHECK(hipMalloc((void**)&ctx->ring.d[k], ProgramRing::kSlotBytes));
@@ -250,16 +255,20 @@ const DevWindow& dev_window(Context& ctx, int norb, int nocc, int lo, int w) {
   return ctx.windows[key] = dw;
 }
 
-const DevPairs& dev_pairs(Context& ctx, int w) {
+const DevGroups& dev_groups(Context& ctx, int w) {
   std::lock_guard<std::mutex> lk(ctx.mu);
-  auto it = ctx.pairs.find(w);
-  if (it != ctx.pairs.end()) return it->second;
-  PairTables T = make_pair_tables(w);
-  DevPairs dp;
-  dp.pairs = upload_table(T.pairs);
-  dp.index = upload_table(T.index);
-  dp.w = w;
-  return ctx.pairs[w] = dp;
+  auto it = ctx.groups.find(w);
+  if (it != ctx.groups.end()) return it->second;
+  GroupTables T = make_group_tables(w, kMaxSubWindow);
+  DevGroups dg;
+  dg.desc = upload_table(T.desc);
+  dg.members = upload_table(T.members);
+  dg.index = upload_table(T.index);
+  dg.host_index = T.index;
+  dg.w = w;
+  dg.w2 = T.w2;
+  dg.n_lo2 = T.n_lo2;
+  return ctx.groups[w] = dg;
 }
 
 // A call's program: host bytes appended, device address known after commit.
@@ -303,6 +312,9 @@ struct Program {
 
 size_t elem_bytes(int dtype) { return dtype == FFSIM_C64 ? 8 : 16; }
 
+// LDS kept free for the staged program of a pass (sub-passes + steps).
+constexpr size_t kProgramLdsReserve = 20480;
+
 int tile_width(int dtype) {
   static int env_w = [] {
     const char* e = std::getenv("FFSIM_AMD_TILE_W");
@@ -326,7 +338,7 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   const size_t per_row = (size_t)(W + 1) * elem_bytes(dtype);
   int best = std::min(norb, 2);
   for (int w = std::min(norb, 16); w >= 2; --w) {
-    if ((size_t)max_fiber(norb, nocc, w) * per_row <= ctx.lds_max) {
+    if ((size_t)max_fiber(norb, nocc, w) * per_row + kProgramLdsReserve <= ctx.lds_max) {
       best = w;
       break;
     }
@@ -335,17 +347,23 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   return best;
 }
 
-template <typename T, int W, int MODE>
+template <typename T, int W, int MODE, int W2>
 void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
-                    const DevWindow& win, bool all_fibers, const DevPairs& pt,
-                    const KStep* d_steps, int n_steps, const double* d_diag,
+                    const DevWindow& win, bool all_fibers, const DevGroups& gr, const LoSlots& los,
+                    int gd_cap, int gm_cap, const SubPass* d_subs, int n_sub,
+                    const LayerPair* d_layers, int n_layers, const double* d_diag,
                     const uint64_t* d_strings, size_t lds_max, hipStream_t stream) {
+  constexpr int NTHR = (W >= 8) ? 512 : 256;
+  constexpr int S = (W >= 8) ? W + 1 : W;
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
   if (n_f == 0 || n_batch == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
-  const size_t lds = (size_t)rows * (W + 1) * sizeof(Cx<T>);
+  const size_t lds = align16((size_t)rows * S * sizeof(Cx<T>)) + align16((size_t)rows * 4) +
+                     (size_t)n_layers * sizeof(LayerPair) + (size_t)n_sub * sizeof(SubPass) +
+                     kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
+                     (size_t)gm_cap * 2;
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
-  auto kern = fiber_kernel<T, W, MODE>;
+  auto kern = fiber_kernel<T, W, MODE, NTHR, W2>;
   static bool attr_set = false;
   if (!attr_set) {
     HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -353,57 +371,86 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
     attr_set = true;
   }
   const int n_bt = (n_batch + W - 1) / W;
-  const long grid = (long)n_bt * n_f;
-  require(grid < (1l << 31), FFSIM_ERR_VALUE, "grid too large");
-  double moved = 0.0;  // algorithmic bytes: every element of every launched fiber read + written
-  {
-    // fibers are sorted with the active ones first; rows per fiber are uniform per kp
-    moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch * (double)(all_fibers ? win.rows_all : win.rows_active);
-  }
+  const long n_tiles = (long)n_bt * n_f;
+  require(n_tiles < (1l << 40), FFSIM_ERR_VALUE, "grid too large");
+  // persistent grid: as many blocks as fit on the device at once
+  const long per_cu = std::max<long>(1, std::min<long>((long)(lds_max / lds), 2048 / NTHR));
+  const long grid = std::min<long>(n_tiles, per_cu * g_n_cu);
+  // algorithmic bytes: every element of every launched fiber read once and written once
+  const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch *
+                       (double)(all_fibers ? win.rows_all : win.rows_active);
   const long pk = prof_start(PROF_FIBER, moved, stream);
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), lds, stream,
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
-                     ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, pt.pairs,
-                     pt.index, std::max(pt.w - 1, 1), d_steps, n_steps, d_diag, d_strings);
+                     ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(), los,
+                     d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, rows, gd_cap, gm_cap);
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
 }
 
 template <typename T>
 void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst, int n_batch,
-                  bool tstore, const DevWindow& win, bool all_fibers, const DevPairs& pt,
-                  const KStep* d_steps, int n_steps, const double* d_diag,
-                  const uint64_t* d_strings, size_t lds_max, int W, hipStream_t stream) {
+                  bool tstore, const DevWindow& win, bool all_fibers, const DevGroups& gr,
+                  const LoSlots& los, int gd_cap, int gm_cap,
+                  const SubPass* d_subs, int n_sub, const LayerPair* d_layers, int n_layers,
+                  const double* d_diag, const uint64_t* d_strings, size_t lds_max, int W,
+                  hipStream_t stream) {
+  // gr.w2 < kMaxSubWindow only for windows narrower than the sub-window
+#define FFSIM_FIBER_LAUNCH(WW, MM, W2)                                                          \
+  launch_fiber_t<T, WW, MM, W2>(src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,  \
+                                los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag,  \
+                                d_strings, lds_max, stream)
 #define FFSIM_FIBER_CASE(WW)                                                                    \
   if (W == WW) {                                                                               \
-    if (mode == MODE_ROWS)                                                                     \
-      launch_fiber_t<T, WW, MODE_ROWS>(src, dst, ld_src, ld_dst, n_batch, tstore, win,         \
-                                       all_fibers, pt, d_steps, n_steps, d_diag, d_strings,    \
-                                       lds_max, stream);                                       \
-    else                                                                                       \
-      launch_fiber_t<T, WW, MODE_COLS>(src, dst, ld_src, ld_dst, n_batch, tstore, win,         \
-                                       all_fibers, pt, d_steps, n_steps, d_diag, d_strings,    \
-                                       lds_max, stream);                                       \
-    return;                                                                                    \
+    if (gr.w2 == kMaxSubWindow) {                                                              \
+      if (mode == MODE_ROWS)                                                                   \
+        FFSIM_FIBER_LAUNCH(WW, MODE_ROWS, kMaxSubWindow);                                      \
+      else                                                                                     \
+        FFSIM_FIBER_LAUNCH(WW, MODE_COLS, kMaxSubWindow);                                      \
+      return;                                                                                  \
+    }                                                                                          \
   }
+  // production widths with the full sub-window
   FFSIM_FIBER_CASE(4)
   FFSIM_FIBER_CASE(8)
   FFSIM_FIBER_CASE(16)
+  // windows narrower than the sub-window (norb < 6): one width suffices
+  if (sizeof(T) == 8) W = 8; else W = 16;
+  if (W == 8) {
+    switch (gr.w2) {
+      case 2: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(8, MODE_ROWS, 2); else FFSIM_FIBER_LAUNCH(8, MODE_COLS, 2); return;
+      case 3: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(8, MODE_ROWS, 3); else FFSIM_FIBER_LAUNCH(8, MODE_COLS, 3); return;
+      case 4: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(8, MODE_ROWS, 4); else FFSIM_FIBER_LAUNCH(8, MODE_COLS, 4); return;
+      default: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(8, MODE_ROWS, 5); else FFSIM_FIBER_LAUNCH(8, MODE_COLS, 5); return;
+    }
+  } else {
+    switch (gr.w2) {
+      case 2: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(16, MODE_ROWS, 2); else FFSIM_FIBER_LAUNCH(16, MODE_COLS, 2); return;
+      case 3: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(16, MODE_ROWS, 3); else FFSIM_FIBER_LAUNCH(16, MODE_COLS, 3); return;
+      case 4: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(16, MODE_ROWS, 4); else FFSIM_FIBER_LAUNCH(16, MODE_COLS, 4); return;
+      default: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(16, MODE_ROWS, 5); else FFSIM_FIBER_LAUNCH(16, MODE_COLS, 5); return;
+    }
+  }
 #undef FFSIM_FIBER_CASE
+#undef FFSIM_FIBER_LAUNCH
   throw Failure{FFSIM_ERR_VALUE, "unsupported tile width"};
 }
 
 void fiber_pass(int dtype, int mode, const void* src, void* dst, long ld_src, long ld_dst,
                 int n_batch, bool tstore, const DevWindow& win, bool all_fibers,
-                const DevPairs& pt, const KStep* d_steps, int n_steps, const double* d_diag,
-                const uint64_t* d_strings, Context& ctx, hipStream_t stream) {
+                const DevGroups& gr, const LoSlots& los, int gd_cap, int gm_cap,
+                const SubPass* d_subs, int n_sub, const LayerPair* d_layers,
+                int n_layers, const double* d_diag, const uint64_t* d_strings, Context& ctx,
+                hipStream_t stream) {
   const int W = tile_width(dtype);
   if (dtype == FFSIM_C128)
-    launch_fiber<double>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, pt,
-                         d_steps, n_steps, d_diag, d_strings, ctx.lds_max, W, stream);
+    launch_fiber<double>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
+                         los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, ctx.lds_max, W,
+                         stream);
   else
-    launch_fiber<float>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, pt,
-                        d_steps, n_steps, d_diag, d_strings, ctx.lds_max, W, stream);
+    launch_fiber<float>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
+                        los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, ctx.lds_max, W,
+                        stream);
 }
 
 // One spin sector's orbital rotation, prepared on the host.
@@ -435,7 +482,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
   std::vector<PlannedPass> passes =
       plan_passes(rot.steps, norb, std::max(w, 1), tstore, -1);
   const DevStrings& ds = dev_strings(ctx, norb, nocc);
-  const DevPairs& pt = dev_pairs(ctx, std::max(w, 1));
+  const DevGroups& gr = dev_groups(ctx, std::max(w, 1));
   const bool need_diag = !rot.trivial_diag;
   size_t diag_off = 0;
   if (need_diag) {
@@ -446,21 +493,43 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     }
     diag_off = prog.append(dd.data(), dd.size() * sizeof(double));
   }
+  std::vector<SubPass> subs;
+  std::vector<LayerPair> sub_layers;
   for (const PlannedPass& P : passes) {
     const bool last = P.last;
     const bool ts = last && tstore;
     const bool dg = last && need_diag;
     if (P.steps.empty() && !ts && !dg) continue;
-    const size_t soff = prog.append(P.steps.data(), P.steps.size() * sizeof(KStep));
-    const int n_steps = (int)P.steps.size();
+    plan_sub_passes(P.steps, std::max(w, 1), kMaxSubWindow, subs, sub_layers);
+    const size_t soff = prog.append(sub_layers.data(), sub_layers.size() * sizeof(LayerPair));
+    const size_t poff = prog.append(subs.data(), subs.size() * sizeof(SubPass));
+    const int n_layers = (int)sub_layers.size();
+    const int n_sub = (int)subs.size();
+    LoSlots los{};
+    for (const SubPass& sp : subs) {
+      require(sp.slot < kMaxLoSlots, FFSIM_ERR_VALUE, "too many sub-window starts");
+      los.lo2[sp.slot] = sp.lo2;
+      los.n = std::max(los.n, sp.slot + 1);
+    }
+    int gd_cap = 0, gm_cap = 0;
+    for (int kp = 0; kp <= gr.w; ++kp) {
+      int nd = 0, nm = 0;
+      for (int u = 0; u < los.n; ++u) {
+        const int* ix = gr.host_index.data() + ((size_t)kp * gr.n_lo2 + los.lo2[u]) * 4;
+        nd += ix[1];
+        nm += ix[3];
+      }
+      gd_cap = std::max(gd_cap, nd);
+      gm_cap = std::max(gm_cap, nm);
+    }
     const DevWindow& win = dev_window(ctx, norb, nocc, P.lo, std::max(w, 1));
     void* dst = ts ? tstore_dst : state;
     const long ld_dst = ts ? ld_t : ld;
     const bool all = ts || dg;
-    launches.push_back([=, &ctx, &prog, &win, &pt, &ds]() {
-      fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, pt,
-                 prog.dev<KStep>(soff), n_steps, dg ? prog.dev<double>(diag_off) : nullptr, ds.d,
-                 ctx, stream);
+    launches.push_back([=, &ctx, &prog, &win, &gr, &ds]() {
+      fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, gr, los, gd_cap,
+                 gm_cap, prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff), n_layers,
+                 dg ? prog.dev<double>(diag_off) : nullptr, ds.d, ctx, stream);
     });
   }
 }
@@ -484,7 +553,7 @@ void orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
   const int Wd = tile_width(dtype);
   // beta runs in place with strings on columns when a whole beta vector tiles
   const bool beta_cols =
-      (size_t)dim_b * (Wd + 1) * es <= ctx.lds_max && (size_t)dim_b < 65536;
+      (size_t)dim_b * (Wd + 1) * es + kProgramLdsReserve <= ctx.lds_max && (size_t)dim_b < 65536;
   Program prog(ctx, stream);
   std::vector<std::function<void()>> launches;
   const bool beta_transposed = mat_b && !beta_cols;
@@ -717,7 +786,8 @@ int64_t ffsim_orbital_rotation_work_elements(int dtype, int norb, int n_alpha, i
     check_sizes(norb, n_alpha, n_beta);
     Context& ctx = context();
     const long dim_a = (long)binom(norb, n_alpha), dim_b = (long)binom(norb, n_beta);
-    const bool beta_cols = (size_t)dim_b * (tile_width(dtype) + 1) * elem_bytes(dtype) <= ctx.lds_max &&
+    const bool beta_cols = (size_t)dim_b * (tile_width(dtype) + 1) * elem_bytes(dtype) +
+                                   kProgramLdsReserve <= ctx.lds_max &&
                            (size_t)dim_b < 65536;
     out = beta_cols ? 0 : (int64_t)dim_a * dim_b;
   });

@@ -143,6 +143,108 @@ PairTables make_pair_tables(int w) {
   return T;
 }
 
+GroupTables make_group_tables(int w, int w2) {
+  GroupTables T;
+  T.w = w;
+  T.w2 = std::min(w2, w);
+  T.n_lo2 = std::max(w - T.w2 + 1, 1);
+  T.index.assign((size_t)(w + 1) * T.n_lo2 * 4, 0);
+  if (T.w2 < 2) return T;
+  const uint64_t sub_bits = (1ull << T.w2) - 1;
+  for (int kp = 0; kp <= w; ++kp) {
+    std::vector<uint64_t> locals = make_strings(w, kp);
+    for (int lo2 = 0; lo2 + T.w2 <= w; ++lo2) {
+      const uint64_t sub_mask = sub_bits << lo2;
+      // distinct outside patterns in ascending order of first appearance
+      std::vector<uint64_t> outs;
+      for (uint64_t s : locals) {
+        uint64_t o = s & ~sub_mask;
+        if (std::find(outs.begin(), outs.end(), o) == outs.end()) outs.push_back(o);
+      }
+      struct G {
+        uint64_t out;
+        int k2;
+      };
+      std::vector<G> gs;
+      for (uint64_t o : outs) {
+        int k2 = kp - __builtin_popcountll(o);
+        if (k2 > 0 && k2 < T.w2) gs.push_back({o, k2});
+      }
+      std::stable_sort(gs.begin(), gs.end(), [&](const G& a, const G& b) {
+        return binom(T.w2, a.k2) > binom(T.w2, b.k2);
+      });
+      const int off = (int)T.desc.size();
+      const int moff0 = (int)T.members.size();
+      for (const G& g : gs) {
+        const uint32_t moff = (uint32_t)T.members.size();
+        for (uint64_t sl : make_strings(T.w2, g.k2))
+          T.members.push_back((uint16_t)string_rank(g.out | (sl << lo2)));
+        T.desc.push_back((moff << 8) | (uint32_t)g.k2);
+      }
+      T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 0] = off;
+      T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 1] = (int)gs.size();
+      T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 2] = moff0;
+      T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 3] = (int)T.members.size() - moff0;
+    }
+  }
+  return T;
+}
+
+void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
+                     std::vector<SubPass>& subs, std::vector<LayerPair>& layers) {
+  subs.clear();
+  layers.clear();
+  if (window_steps.empty()) return;
+  w2 = std::min(w2, w);
+  std::vector<PlannedPass> sp = plan_passes(window_steps, w, w2, false, -1);
+  for (const PlannedPass& P : sp) {
+    if (P.steps.empty()) continue;
+    // ASAP layering with layer parity fixed by the pair position
+    std::vector<int> last(w2 + 1, -1);
+    std::vector<int> lay(P.steps.size());
+    int n_layers = 0;
+    for (size_t r = 0; r < P.steps.size(); ++r) {
+      const int p = P.steps[r].p;
+      int L = std::max(last[p], last[p + 1]) + 1;
+      if ((L & 1) != (p & 1)) ++L;
+      lay[r] = L;
+      last[p] = last[p + 1] = L;
+      n_layers = std::max(n_layers, L + 1);
+    }
+    const int n_lp = (n_layers + 1) / 2;
+    SubPass s{};
+    s.lo2 = P.lo;
+    s.lp0 = (int)layers.size();
+    s.n_lp = n_lp;
+    s.slot = -1;
+    for (const SubPass& o : subs)
+      if (o.lo2 == P.lo) s.slot = o.slot;
+    if (s.slot < 0) {
+      int n_slots = 0;
+      for (const SubPass& o : subs) n_slots = std::max(n_slots, o.slot + 1);
+      s.slot = n_slots;
+    }
+    LayerPair id{};
+    for (int k = 0; k < kSlots; ++k) {
+      id.c[k] = 1.0;
+      id.sr[k] = 0.0;
+      id.si[k] = 0.0;
+    }
+    id.mask = 0;
+    layers.insert(layers.end(), n_lp, id);
+    for (size_t r = 0; r < P.steps.size(); ++r) {
+      const KStep& k = P.steps[r];
+      LayerPair& L = layers[s.lp0 + lay[r] / 2];
+      if (L.mask & (1 << k.p)) throw std::logic_error("layer slot used twice");
+      L.c[k.p] = k.c;
+      L.sr[k.p] = k.sr;
+      L.si[k.p] = k.si;
+      L.mask |= 1 << k.p;
+    }
+    subs.push_back(s);
+  }
+}
+
 // ------------------------------------------------------------------ Givens
 namespace {
 // Rotation that zeroes b against a: returns (c, s) with c real such that

@@ -60,6 +60,24 @@ struct PairTables {
 };
 PairTables make_pair_tables(int w);
 
+// Register groups of a sub-window [lo2, lo2 + w2) inside a window of width w:
+// for every kp and every sub-window start lo2 in [0, w - w2], the window-local
+// strings with kp electrons are split into groups that agree outside the
+// sub-window.  A group with k2 electrons inside the sub-window has C(w2, k2)
+// members, listed in ascending order of their sub-window bits, so member m is
+// the m-th k2-subset of w2 bits.  Groups with k2 == 0 or k2 == w2 are not
+// stored (no rotation inside the sub-window changes them).  Groups are sorted
+// by decreasing size.
+struct GroupTables {
+  int w = 0, w2 = 0, n_lo2 = 0;
+  std::vector<uint32_t> desc;     // member_offset << 8 | k2
+  std::vector<uint16_t> members;  // window-local ranks
+  // [(kp * n_lo2 + lo2) * 4 + {0: desc offset, 1: group count, 2: member offset,
+  //                           3: member count}]; a (kp, lo2) set is contiguous
+  std::vector<int32_t> index;
+};
+GroupTables make_group_tables(int w, int w2);
+
 // ------------------------------------------------------ Givens decomposition
 // Exact Clements decomposition U = D G_L^* ... G_1^* with each G acting on
 // adjacent orbitals (i, j) as [[c, s], [-conj(s), c]] (c real), matching the
@@ -98,6 +116,31 @@ struct PlannedPass {
   std::vector<KStep> steps;  // window-local p
   bool last = false;         // final pass of the network
 };
+// A window pass is split into register sub-passes over sub-windows of width
+// w2 <= kMaxSubWindow.  Inside a sub-pass the rotations are packed into
+// brickwork layer pairs: an even layer (pairs p = 0, 2, 4 of the sub-window)
+// followed by an odd layer (p = 1, 3); slot p of a layer pair holds the
+// rotation on (p, p + 1) or is empty (bit p of `mask` clear).
+constexpr int kMaxSubWindow = 6;
+constexpr int kSlots = kMaxSubWindow - 1;
+
+struct LayerPair {
+  double c[kSlots], sr[kSlots], si[kSlots];
+  int32_t mask;
+  int32_t pad;
+};
+static_assert(sizeof(LayerPair) == 128, "LayerPair layout is shared with the device");
+
+struct SubPass {
+  int lo2;
+  int lp0, n_lp;  // range in the pass's layer-pair list
+  int slot;       // index of lo2 among the distinct sub-window starts of the pass
+};
+// Splits the steps of one window pass (window-local p) into sub-passes and
+// packs each into layer pairs (ASAP on every orbital, layer parity = p parity).
+void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
+                     std::vector<SubPass>& subs, std::vector<LayerPair>& layers);
+
 // Greedy window schedule: alternate between the windows [0, w) and
 // [norb - w, norb), each pass applying every rotation whose pair lies inside
 // the window and whose predecessors (earlier rotations sharing an orbital)

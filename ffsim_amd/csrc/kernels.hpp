@@ -25,76 +25,272 @@ struct alignas(2 * sizeof(T)) Cx {
 
 enum { MODE_ROWS = 0, MODE_COLS = 1 };
 
+// ---------------------------------------------------------------------------
+// Compile-time pair lists of a sub-window: among the k2-subsets of w2 bits in
+// ascending order (member index = rank), the pairs (x, y) where x has bit p
+// set / p+1 clear and y = x with the electron moved from p to p+1.
+struct SubPairList {
+  int n;
+  unsigned char x[16], y[16];
+};
+
+constexpr int popc_const(unsigned v) {
+  int c = 0;
+  while (v) {
+    c += v & 1u;
+    v >>= 1;
+  }
+  return c;
+}
+
+constexpr int rank_const(unsigned s, int w2, int k2) {
+  int r = 0;
+  for (unsigned t = 0; t < s; ++t)
+    if (popc_const(t) == k2) ++r;
+  (void)w2;
+  return r;
+}
+
+template <int W2, int K2, int P>
+constexpr SubPairList make_sub_pairs() {
+  SubPairList pl{};
+  pl.n = 0;
+  int r = 0;
+  for (unsigned s = 0; s < (1u << W2); ++s) {
+    if (popc_const(s) != K2) continue;
+    if (((s >> P) & 1u) && !((s >> (P + 1)) & 1u)) {
+      pl.x[pl.n] = (unsigned char)r;
+      pl.y[pl.n] = (unsigned char)rank_const(s ^ (3u << P), W2, K2);
+      ++pl.n;
+    }
+    ++r;
+  }
+  return pl;
+}
+
+constexpr int binom_const(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  int r = 1;
+  for (int t = 1; t <= k; ++t) r = r * (n - k + t) / t;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void rot_pair(Cx<T>& a, Cx<T>& b, T c, T sr, T si) {
+  const Cx<T> x = a, y = b;
+  a.re = c * x.re + (sr * y.re - si * y.im);
+  a.im = c * x.im + (sr * y.im + si * y.re);
+  b.re = c * y.re - (sr * x.re + si * x.im);
+  b.im = c * y.im - (sr * x.im - si * x.re);
+}
+
+template <typename T, int W2, int K2, int P>
+__device__ __forceinline__ void rot_sub(Cx<T>* v, T c, T sr, T si) {
+  if constexpr (P + 1 < W2) {
+    constexpr SubPairList pl = make_sub_pairs<W2, K2, P>();
+#pragma unroll
+    for (int q = 0; q < pl.n; ++q) rot_pair(v[pl.x[q]], v[pl.y[q]], c, sr, si);
+  }
+}
+
+template <typename T, int W2, int K2, int P>
+__device__ __forceinline__ void slot_apply(Cx<T>* v, const LayerPair& L, int mask) {
+  if constexpr (P + 1 < W2) {
+    if (mask & (1 << P)) rot_sub<T, W2, K2, P>(v, (T)L.c[P], (T)L.sr[P], (T)L.si[P]);
+  }
+}
+
+// Applies the layer pairs of one sub-pass to one register group (one batch column).
+template <typename T, int W2, int K2>
+__device__ __forceinline__ void group_apply(Cx<T>* tile, int S, int j,
+                                            const uint16_t* __restrict__ members,
+                                            const LayerPair* lp, int n_lp) {
+  constexpr int N = binom_const(W2, K2);
+  Cx<T> v[N];
+  int idx[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) idx[m] = (int)members[m] * S + j;
+#pragma unroll
+  for (int m = 0; m < N; ++m) v[m] = tile[idx[m]];
+  for (int l = 0; l < n_lp; ++l) {
+    const LayerPair& L = lp[l];
+    const int mask = __builtin_amdgcn_readfirstlane(L.mask);
+    slot_apply<T, W2, K2, 0>(v, L, mask);
+    slot_apply<T, W2, K2, 2>(v, L, mask);
+    slot_apply<T, W2, K2, 4>(v, L, mask);
+    slot_apply<T, W2, K2, 1>(v, L, mask);
+    slot_apply<T, W2, K2, 3>(v, L, mask);
+  }
+#pragma unroll
+  for (int m = 0; m < N; ++m) tile[idx[m]] = v[m];
+}
+
+template <typename T, int W2>
+__device__ __forceinline__ void group_dispatch(int k2, Cx<T>* tile, int S, int j,
+                                               const uint16_t* members, const LayerPair* lp,
+                                               int n_lp) {
+  switch (k2) {
+    case 1:
+      if constexpr (W2 >= 2) group_apply<T, W2, 1>(tile, S, j, members, lp, n_lp);
+      break;
+    case 2:
+      if constexpr (W2 >= 3) group_apply<T, W2, 2>(tile, S, j, members, lp, n_lp);
+      break;
+    case 3:
+      if constexpr (W2 >= 4) group_apply<T, W2, 3>(tile, S, j, members, lp, n_lp);
+      break;
+    case 4:
+      if constexpr (W2 >= 5) group_apply<T, W2, 4>(tile, S, j, members, lp, n_lp);
+      break;
+    case 5:
+      if constexpr (W2 >= 6) group_apply<T, W2, 5>(tile, S, j, members, lp, n_lp);
+      break;
+    default:
+      break;
+  }
+}
+
+// Program of one window pass, as stored in device memory:
+//   SubPass subs[n_sub]; LayerPair layers[n_layers] (slots relative to each lo2).
+struct GroupArgs {
+  const uint32_t* desc;
+  const uint16_t* members;
+  const int32_t* index;
+  int n_lo2, w2;
+};
+
+// Distinct sub-window starts of a pass (a SubPass's `slot` indexes these).
+constexpr int kMaxLoSlots = 8;
+struct LoSlots {
+  int n;
+  int lo2[kMaxLoSlots];
+};
+
+// Per-block view of one sub-window start's group table, staged in LDS.
+struct GroupSlot {
+  int d_off;  // first descriptor in LDS
+  int nd;     // number of groups
+  int pad[2];
+};
+
+__host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
 // Element (string i, batch j) of a fiber tile lives at
 //   MODE_ROWS: state[row_table[i] * ld + b0 + j]   (strings index rows)
 //   MODE_COLS: state[(b0 + j) * ld + row_table[i]] (strings index columns)
 // tstore (MODE_ROWS only): write element (i, j) to dst[(b0 + j) * ld_dst +
 // row_table[i]] instead, i.e. store the transposed state.
-template <typename T, int W, int MODE>
-__global__ void __launch_bounds__(256)
-    fiber_kernel(const Cx<T>* src, Cx<T>* dst, long ld_src, long ld_dst,
-                 int n_batch, int tstore, const Fiber* __restrict__ fibers, int n_fibers,
-                 const int32_t* __restrict__ row_table, const uint32_t* __restrict__ pair_tab,
-                 const int32_t* __restrict__ pair_idx, int wm1, const KStep* __restrict__ steps,
-                 int n_steps, const double* __restrict__ diag,
-                 const uint64_t* __restrict__ strings) {
+//
+// Persistent: block b handles tiles b, b + gridDim.x, ... in fiber-major
+// order (tile = fiber * n_bt + batch tile), so consecutive tiles of a block
+// share their fiber and its staged tables.  While a tile is transformed in
+// LDS, the next tile's loads are already in flight into registers.
+//
+// LDS: tile[max_rows * S] | rows[max_rows] | layers | subs | slots | desc | members
+template <typename T, int W, int MODE, int NTHR, int W2>
+__global__ void __launch_bounds__(NTHR)
+    fiber_kernel(const Cx<T>* src, Cx<T>* dst, long ld_src, long ld_dst, int n_batch, int tstore,
+                 const Fiber* __restrict__ fibers, int n_fibers,
+                 const int32_t* __restrict__ row_table, GroupArgs ga, LoSlots los,
+                 const SubPass* __restrict__ subs_g, int n_sub,
+                 const LayerPair* __restrict__ layers_g, int n_layers,
+                 const double* __restrict__ diag, const uint64_t* __restrict__ strings,
+                 int max_rows, int gd_cap, int gm_cap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int S = (W >= 8) ? W + 1 : W;  // LDS stride of one string's batch
+  constexpr int RSTEP = NTHR / W;          // rows per sweep in MODE_ROWS
+  constexpr int CH = 16;                   // register chunk of one thread
+  size_t off = 0;
   Cx<T>* tile = reinterpret_cast<Cx<T>*>(smem_raw);
-  constexpr int S = W + 1;  // padded LDS stride (conflict-free column walks)
-  const int f = blockIdx.x % n_fibers;
-  const int bt = blockIdx.x / n_fibers;
-  const Fiber fd = fibers[f];
-  const int nr = fd.n_rows;
-  const int b0 = bt * W;
-  const int nb = min(W, n_batch - b0);
-  const int32_t* rows = row_table + fd.row_off;
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
+  off += align16((size_t)max_rows * S * sizeof(Cx<T>));
+  int32_t* rows = reinterpret_cast<int32_t*>(smem_raw + off);
+  off += align16((size_t)max_rows * sizeof(int32_t));
+  LayerPair* layers = reinterpret_cast<LayerPair*>(smem_raw + off);
+  off += (size_t)n_layers * sizeof(LayerPair);
+  SubPass* subs = reinterpret_cast<SubPass*>(smem_raw + off);
+  off += (size_t)n_sub * sizeof(SubPass);
+  GroupSlot* gslot = reinterpret_cast<GroupSlot*>(smem_raw + off);
+  off += (size_t)kMaxLoSlots * sizeof(GroupSlot);
+  uint32_t* gdesc = reinterpret_cast<uint32_t*>(smem_raw + off);
+  off += align16((size_t)gd_cap * sizeof(uint32_t));
+  uint16_t* gmem = reinterpret_cast<uint16_t*>(smem_raw + off);
+  (void)gm_cap;
 
-  // ---- load the tile
-  if (MODE == MODE_ROWS) {
-    const int j = tid % W;
-    for (int i = tid / W; i < nr; i += nthr / W) {
-      Cx<T> v = {T(0), T(0)};
-      if (j < nb) v = src[(long)rows[i] * ld_src + b0 + j];
-      tile[i * S + j] = v;
-    }
-  } else {
-    for (int j = 0; j < W; ++j) {
-      for (int i = tid; i < nr; i += nthr) {
-        Cx<T> v = {T(0), T(0)};
-        if (j < nb) v = src[(long)(b0 + j) * ld_src + rows[i]];
-        tile[i * S + j] = v;
+  const int tid = threadIdx.x;
+  const int n_bt = (n_batch + W - 1) / W;
+  const long n_tiles = (long)n_bt * n_fibers;
+
+  for (int k = tid; k < n_layers; k += NTHR) layers[k] = layers_g[k];
+  for (int k = tid; k < n_sub; k += NTHR) subs[k] = subs_g[k];
+
+  // stage the row addresses and group tables of fiber f
+  auto stage_fiber = [&](const Fiber& fd) {
+    for (int i = tid; i < fd.n_rows; i += NTHR) rows[i] = row_table[fd.row_off + i];
+    if (n_sub > 0) {
+      int d_base = 0, m_base = 0;
+      for (int u = 0; u < los.n; ++u) {
+        const int* ix = ga.index + (fd.kp * ga.n_lo2 + los.lo2[u]) * 4;
+        const int d0 = ix[0], nd = ix[1], m0 = ix[2], nm = ix[3];
+        if (tid == 0) gslot[u] = GroupSlot{d_base, nd, {0, 0}};
+        for (int k = tid; k < nd; k += NTHR) {
+          const uint32_t d = ga.desc[d0 + k];
+          gdesc[d_base + k] = ((((d >> 8) - (uint32_t)m0) + (uint32_t)m_base) << 8) | (d & 0xff);
+        }
+        for (int k = tid; k < nm; k += NTHR) gmem[m_base + k] = ga.members[m0 + k];
+        d_base += nd;
+        m_base += nm;
       }
     }
-  }
-  __syncthreads();
-
-  // ---- Givens program
-  for (int s = 0; s < n_steps; ++s) {
-    const KStep st = steps[s];
-    const int pi = (fd.kp * wm1 + st.p) * 2;
-    const int off = pair_idx[pi];
-    const int cnt = pair_idx[pi + 1];
-    const T c = (T)st.c, sr = (T)st.sr, si = (T)st.si;
-    for (int e = tid; e < cnt * W; e += nthr) {
-      const int q = e / W, j = e % W;
-      const uint32_t pr = pair_tab[off + q];
-      const int x = pr & 0xffff, y = pr >> 16;
-      const Cx<T> a = tile[x * S + j];
-      const Cx<T> b = tile[y * S + j];
-      Cx<T> na, nb2;
-      na.re = c * a.re + (sr * b.re - si * b.im);
-      na.im = c * a.im + (sr * b.im + si * b.re);
-      nb2.re = c * b.re - (sr * a.re + si * a.im);
-      nb2.im = c * b.im - (sr * a.im - si * a.re);
-      tile[x * S + j] = na;
-      tile[y * S + j] = nb2;
+  };
+  // issue every global load of tile (nr rows, batch offset b0, nb columns) into buf
+  auto load_regs = [&](Cx<T>* buf, int nr, int b0, int nb) {
+    if (MODE == MODE_ROWS) {
+      const int j = tid % W, i0 = tid / W;
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int i = i0 + k * RSTEP;
+        buf[k] = Cx<T>{T(0), T(0)};
+        if (i < nr && j < nb) buf[k] = src[(long)rows[i] * ld_src + b0 + j];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int i = tid + (k / W) * NTHR, j = k % W;
+        buf[k] = Cx<T>{T(0), T(0)};
+        if (i < nr && j < nb) buf[k] = src[(long)(b0 + j) * ld_src + rows[i]];
+      }
     }
-    __syncthreads();
-  }
-
-  // ---- optional diagonal phases D (orbital rotation tail) and store
+  };
+  auto regs_to_lds = [&](const Cx<T>* buf, int nr) {
+    if (MODE == MODE_ROWS) {
+      const int j = tid % W, i0 = tid / W;
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int i = i0 + k * RSTEP;
+        if (i < nr) tile[i * S + j] = buf[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int i = tid + (k / W) * NTHR, j = k % W;
+        if (i < nr) tile[i * S + j] = buf[k];
+      }
+    }
+  };
+  // rows not covered by one register chunk (only for fibers above CH * RSTEP rows)
+  auto load_rest = [&](int nr, int b0, int nb) {
+    const int covered = (MODE == MODE_ROWS) ? CH * RSTEP : (CH / W) * NTHR;
+    if (nr <= covered) return;
+    if (MODE == MODE_ROWS) {
+      const int j = tid % W, i0 = tid / W;
+      for (int i = covered + i0; i < nr; i += RSTEP)
+        tile[i * S + j] = (j < nb) ? src[(long)rows[i] * ld_src + b0 + j] : Cx<T>{T(0), T(0)};
+    } else {
+      for (int i = covered + tid; i < nr; i += NTHR)
+        for (int j = 0; j < W; ++j)
+          tile[i * S + j] = (j < nb) ? src[(long)(b0 + j) * ld_src + rows[i]] : Cx<T>{T(0), T(0)};
+    }
+  };
   auto row_phase = [&](int i, T& pr, T& pim) {
     double re = 1.0, im = 0.0;
     uint64_t bits = strings[rows[i]];
@@ -109,36 +305,98 @@ __global__ void __launch_bounds__(256)
     pr = (T)re;
     pim = (T)im;
   };
-  if (MODE == MODE_ROWS && !tstore) {
-    const int j = tid % W;
-    if (j < nb) {
-      for (int i = tid / W; i < nr; i += nthr / W) {
-        Cx<T> v = tile[i * S + j];
-        if (diag) {
-          T pr, pim;
-          row_phase(i, pr, pim);
-          const T t = v.re * pr - v.im * pim;
-          v.im = v.re * pim + v.im * pr;
-          v.re = t;
+  auto store_tile = [&](int nr, int b0, int nb) {
+    if (MODE == MODE_ROWS && !tstore) {
+      const int j = tid % W, i0 = tid / W;
+      if (j < nb) {
+#pragma unroll 4
+        for (int i = i0; i < nr; i += RSTEP) {
+          Cx<T> v = tile[i * S + j];
+          if (diag) {
+            T pr, pim;
+            row_phase(i, pr, pim);
+            const T t = v.re * pr - v.im * pim;
+            v.im = v.re * pim + v.im * pr;
+            v.re = t;
+          }
+          dst[(long)rows[i] * ld_dst + b0 + j] = v;
         }
-        dst[(long)rows[i] * ld_dst + b0 + j] = v;
+      }
+    } else {
+      for (int i = tid; i < nr; i += NTHR) {
+        T pr = T(1), pim = T(0);
+        if (diag) row_phase(i, pr, pim);
+        const long r = rows[i];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          if (j < nb) {
+            Cx<T> v = tile[i * S + j];
+            if (diag) {
+              const T t = v.re * pr - v.im * pim;
+              v.im = v.re * pim + v.im * pr;
+              v.re = t;
+            }
+            dst[(long)(b0 + j) * ld_dst + r] = v;
+          }
+        }
       }
     }
-  } else {
-    for (int i = tid; i < nr; i += nthr) {
-      T pr = T(1), pim = T(0);
-      if (diag) row_phase(i, pr, pim);
-      const long r = rows[i];
-      for (int j = 0; j < nb; ++j) {
-        Cx<T> v = tile[i * S + j];
-        if (diag) {
-          const T t = v.re * pr - v.im * pim;
-          v.im = v.re * pim + v.im * pr;
-          v.re = t;
-        }
-        dst[(long)(b0 + j) * ld_dst + r] = v;
+  };
+
+  long t = blockIdx.x;
+  if (t >= n_tiles) return;
+  int f = (int)(t / n_bt);
+  Fiber fd = fibers[f];
+  stage_fiber(fd);
+  __syncthreads();
+  Cx<T> buf[CH];
+  {
+    const int b0 = (int)(t % n_bt) * W;
+    const int nb = min(W, n_batch - b0);
+    load_regs(buf, fd.n_rows, b0, nb);
+    regs_to_lds(buf, fd.n_rows);
+    load_rest(fd.n_rows, b0, nb);
+  }
+  while (true) {
+    const int nr = fd.n_rows;
+    const int b0 = (int)(t % n_bt) * W;
+    const int nb = min(W, n_batch - b0);
+    const long tn = t + gridDim.x;
+    const int fn = tn < n_tiles ? (int)(tn / n_bt) : -1;
+    const bool prefetch = (fn == f);
+    const int b0n = (int)(tn % n_bt) * W;
+    const int nbn = prefetch ? min(W, n_batch - b0n) : 0;
+    if (prefetch) load_regs(buf, nr, b0n, nbn);
+
+    // ---- register sub-passes on the current tile
+    for (int sp = 0; sp < n_sub; ++sp) {
+      __syncthreads();
+      const SubPass P = subs[sp];
+      const GroupSlot gs = gslot[P.slot];
+      const LayerPair* lp = layers + P.lp0;
+      for (int item = tid; item < gs.nd * W; item += NTHR) {
+        const int g = item / W, j = item % W;
+        const uint32_t d = gdesc[gs.d_off + g];
+        group_dispatch<T, W2>(d & 0xff, tile, S, j, gmem + (d >> 8), lp, P.n_lp);
       }
     }
+    __syncthreads();
+    store_tile(nr, b0, nb);
+    __syncthreads();
+    if (tn >= n_tiles) break;
+    if (prefetch) {
+      regs_to_lds(buf, nr);
+      load_rest(nr, b0n, nbn);
+    } else {
+      f = fn;
+      fd = fibers[f];
+      stage_fiber(fd);
+      __syncthreads();
+      load_regs(buf, fd.n_rows, b0n, min(W, n_batch - b0n));
+      regs_to_lds(buf, fd.n_rows);
+      load_rest(fd.n_rows, b0n, min(W, n_batch - b0n));
+    }
+    t = tn;
   }
 }
 

@@ -112,13 +112,21 @@ struct DevWindow {
   int n_fibers = 0, n_active = 0, max_rows = 0, max_rows_active = 0, lo = 0, w = 0;
   long rows_all = 0, rows_active = 0;  // strings covered by all / by the active fibers
 };
+struct DevSchedule {
+  uint32_t* chunks = nullptr;
+  int32_t* index = nullptr;
+};
 struct DevGroups {
   uint32_t* desc = nullptr;
   uint16_t* members = nullptr;
   int32_t* index = nullptr;
   std::vector<int32_t> host_index;  // copy of index for LDS sizing on the host
+  GroupTables host;                 // host tables (wave schedules are built from them)
+  std::map<std::pair<int, int>, DevSchedule> schedules;  // (n_waves, W)
   int w = 0, w2 = 0, n_lo2 = 0;
-  GroupArgs args() const { return GroupArgs{desc, members, index, n_lo2, w2}; }
+  GroupArgs args(const DevSchedule& ds) const {
+    return GroupArgs{desc, members, index, n_lo2, w2, ds.chunks, ds.index};
+  }
 };
 
 // Program ring: per-call gate parameters (Givens steps, phases, diagonal
@@ -265,10 +273,24 @@ const DevGroups& dev_groups(Context& ctx, int w) {
   dg.members = upload_table(T.members);
   dg.index = upload_table(T.index);
   dg.host_index = T.index;
+  dg.host = T;
   dg.w = w;
   dg.w2 = T.w2;
   dg.n_lo2 = T.n_lo2;
   return ctx.groups[w] = dg;
+}
+
+const DevSchedule& dev_schedule(Context& ctx, const DevGroups& gr, int n_waves, int W) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  DevGroups& g = const_cast<DevGroups&>(gr);
+  auto key = std::make_pair(n_waves, W);
+  auto it = g.schedules.find(key);
+  if (it != g.schedules.end()) return it->second;
+  WaveSchedule ws = make_wave_schedule(g.host, n_waves, std::max(1, 64 / W));
+  DevSchedule ds;
+  ds.chunks = upload_table(ws.chunks);
+  ds.index = upload_table(ws.index);
+  return g.schedules[key] = ds;
 }
 
 // A call's program: host bytes appended, device address known after commit.
@@ -332,12 +354,21 @@ int max_fiber(int norb, int nocc, int w) {
   return (int)m;
 }
 
-// Largest window whose fiber tiles fit in LDS.
+int window_cap() {
+  static int cap = [] {
+    const char* e = std::getenv("FFSIM_AMD_MAX_WINDOW");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 2 && v <= 16) ? v : 12;
+  }();
+  return cap;
+}
+
+// Largest window (up to the cap) whose fiber tiles fit in LDS.
 int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   const int W = tile_width(dtype);
   const size_t per_row = (size_t)(W + 1) * elem_bytes(dtype);
   int best = std::min(norb, 2);
-  for (int w = std::min(norb, 16); w >= 2; --w) {
+  for (int w = std::min(norb, window_cap()); w >= 2; --w) {
     if ((size_t)max_fiber(norb, nocc, w) * per_row + kProgramLdsReserve <= ctx.lds_max) {
       best = w;
       break;
@@ -347,23 +378,25 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   return best;
 }
 
-template <typename T, int W, int MODE, int W2>
+template <typename T, int W, int MODE, int W2, int NTHR>
 void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
                     const DevWindow& win, bool all_fibers, const DevGroups& gr, const LoSlots& los,
                     int gd_cap, int gm_cap, const SubPass* d_subs, int n_sub,
                     const LayerPair* d_layers, int n_layers, const double* d_diag,
                     const uint64_t* d_strings, size_t lds_max, hipStream_t stream) {
-  constexpr int NTHR = (W >= 8) ? 512 : 256;
   constexpr int S = (W >= 8) ? W + 1 : W;
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
   if (n_f == 0 || n_batch == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
+  const int gc_cap = gd_cap;  // every chunk holds at least one group
   const size_t lds = align16((size_t)rows * S * sizeof(Cx<T>)) + align16((size_t)rows * 4) +
                      (size_t)n_layers * sizeof(LayerPair) + (size_t)n_sub * sizeof(SubPass) +
                      kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
-                     (size_t)gm_cap * 2;
+                     align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * (NTHR / 64) * 8 +
+                     (size_t)gc_cap * 4;
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
   auto kern = fiber_kernel<T, W, MODE, NTHR, W2>;
+  const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
   static bool attr_set = false;
   if (!attr_set) {
     HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -374,7 +407,9 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
   const long n_tiles = (long)n_bt * n_f;
   require(n_tiles < (1l << 40), FFSIM_ERR_VALUE, "grid too large");
   // persistent grid: as many blocks as fit on the device at once
-  const long per_cu = std::max<long>(1, std::min<long>((long)(lds_max / lds), 2048 / NTHR));
+  int occ = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, NTHR, lds));
+  const long per_cu = std::max(1, occ);
   const long grid = std::min<long>(n_tiles, per_cu * g_n_cu);
   // algorithmic bytes: every element of every launched fiber read once and written once
   const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch *
@@ -382,7 +417,7 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
   const long pk = prof_start(PROF_FIBER, moved, stream);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
-                     ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(), los,
+                     ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(sched), los,
                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, rows, gd_cap, gm_cap);
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
@@ -396,10 +431,26 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                   const double* d_diag, const uint64_t* d_strings, size_t lds_max, int W,
                   hipStream_t stream) {
   // gr.w2 < kMaxSubWindow only for windows narrower than the sub-window
-#define FFSIM_FIBER_LAUNCH(WW, MM, W2)                                                          \
-  launch_fiber_t<T, WW, MM, W2>(src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,  \
-                                los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag,  \
-                                d_strings, lds_max, stream)
+#define FFSIM_FIBER_LAUNCH_N(WW, MM, W2, NT)                                                    \
+  launch_fiber_t<T, WW, MM, W2, NT>(src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers,  \
+                                    gr, los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers,  \
+                                    d_diag, d_strings, lds_max, stream)
+  // 512 threads when one block fills the CU's LDS, 256 when two or more fit
+  const int rows_l = all_fibers ? win.max_rows : win.max_rows_active;
+  const int S_l = (W >= 8) ? W + 1 : W;
+  const size_t lds_l = align16((size_t)rows_l * S_l * sizeof(Cx<T>)) + align16((size_t)rows_l * 4) +
+                       (size_t)n_layers * sizeof(LayerPair) + (size_t)n_sub * sizeof(SubPass) +
+                       kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
+                       align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * 16 * 8 +
+                       (size_t)gd_cap * 4;
+  const bool wide = 2 * lds_l > lds_max;
+#define FFSIM_FIBER_LAUNCH(WW, MM, W2)               \
+  do {                                               \
+    if (wide)                                        \
+      FFSIM_FIBER_LAUNCH_N(WW, MM, W2, 512);         \
+    else                                             \
+      FFSIM_FIBER_LAUNCH_N(WW, MM, W2, 256);         \
+  } while (0)
 #define FFSIM_FIBER_CASE(WW)                                                                    \
   if (W == WW) {                                                                               \
     if (gr.w2 == kMaxSubWindow) {                                                              \
@@ -416,6 +467,8 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
   FFSIM_FIBER_CASE(16)
   // windows narrower than the sub-window (norb < 6): one width suffices
   if (sizeof(T) == 8) W = 8; else W = 16;
+#undef FFSIM_FIBER_LAUNCH
+#define FFSIM_FIBER_LAUNCH(WW, MM, W2) FFSIM_FIBER_LAUNCH_N(WW, MM, W2, 256)
   if (W == 8) {
     switch (gr.w2) {
       case 2: if (mode == MODE_ROWS) FFSIM_FIBER_LAUNCH(8, MODE_ROWS, 2); else FFSIM_FIBER_LAUNCH(8, MODE_COLS, 2); return;
@@ -433,6 +486,7 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
   }
 #undef FFSIM_FIBER_CASE
 #undef FFSIM_FIBER_LAUNCH
+#undef FFSIM_FIBER_LAUNCH_N
   throw Failure{FFSIM_ERR_VALUE, "unsupported tile width"};
 }
 

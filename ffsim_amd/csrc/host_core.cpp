@@ -171,7 +171,8 @@ GroupTables make_group_tables(int w, int w2) {
         if (k2 > 0 && k2 < T.w2) gs.push_back({o, k2});
       }
       std::stable_sort(gs.begin(), gs.end(), [&](const G& a, const G& b) {
-        return binom(T.w2, a.k2) > binom(T.w2, b.k2);
+        const uint64_t sa = binom(T.w2, a.k2), sb = binom(T.w2, b.k2);
+        return sa != sb ? sa > sb : a.k2 < b.k2;
       });
       const int off = (int)T.desc.size();
       const int moff0 = (int)T.members.size();
@@ -190,13 +191,116 @@ GroupTables make_group_tables(int w, int w2) {
   return T;
 }
 
+WaveSchedule make_wave_schedule(const GroupTables& T, int n_waves, int per_chunk) {
+  WaveSchedule ws;
+  ws.n_waves = n_waves;
+  ws.index.assign((size_t)(T.w + 1) * T.n_lo2 * n_waves * 2, 0);
+  const int n_simd = std::min(4, n_waves);
+  for (int kp = 0; kp <= T.w; ++kp) {
+    for (int lo2 = 0; lo2 < T.n_lo2; ++lo2) {
+      const int* ix = T.index.data() + ((size_t)kp * T.n_lo2 + lo2) * 4;
+      const int d0 = ix[0], nd = ix[1];
+      struct Ch {
+        int g0, ng;
+        double cost;
+      };
+      std::vector<Ch> chs;
+      for (int g = 0; g < nd;) {
+        const int k2 = T.desc[d0 + g] & 0xff;
+        int e = g;
+        while (e < nd && e - g < per_chunk && (int)(T.desc[d0 + e] & 0xff) == k2) ++e;
+        // rotation work per lane ~ pairs per pair position; plus the group's loads/stores
+        const double pairs = (double)binom(T.w2 - 2, k2 - 1);
+        const double size = (double)binom(T.w2, k2);
+        chs.push_back({g, e - g, 6.0 * pairs + size});
+        g = e;
+      }
+      std::stable_sort(chs.begin(), chs.end(),
+                       [](const Ch& a, const Ch& b) { return a.cost > b.cost; });
+      std::vector<double> simd_load(n_simd, 0.0), wave_load(n_waves, 0.0);
+      std::vector<std::vector<uint32_t>> per_wave(n_waves);
+      for (const Ch& c : chs) {
+        int s = 0;
+        for (int k = 1; k < n_simd; ++k)
+          if (simd_load[k] < simd_load[s]) s = k;
+        int wv = s;
+        for (int k = s; k < n_waves; k += n_simd)
+          if (wave_load[k] < wave_load[wv]) wv = k;
+        simd_load[s] += c.cost;
+        wave_load[wv] += c.cost;
+        per_wave[wv].push_back(((uint32_t)c.g0 << 8) | (uint32_t)c.ng);
+      }
+      for (int wv = 0; wv < n_waves; ++wv) {
+        int32_t* out = ws.index.data() + (((size_t)kp * T.n_lo2 + lo2) * n_waves + wv) * 2;
+        out[0] = (int32_t)ws.chunks.size();
+        out[1] = (int32_t)per_wave[wv].size();
+        ws.chunks.insert(ws.chunks.end(), per_wave[wv].begin(), per_wave[wv].end());
+      }
+    }
+  }
+  return ws;
+}
+
+// Greedy sub-window schedule: at every step take the sub-window start whose
+// closure (rotations inside it with all predecessors applied) is largest.
+std::vector<PlannedPass> plan_greedy_max(const std::vector<KStep>& steps, int n, int w2) {
+  std::vector<PlannedPass> out;
+  if (w2 >= n) {
+    PlannedPass P;
+    P.lo = 0;
+    P.steps = steps;
+    out.push_back(P);
+    return out;
+  }
+  std::vector<char> done(steps.size(), 0);
+  size_t n_done = 0;
+  while (n_done < steps.size()) {
+    int best_lo = -1;
+    std::vector<size_t> best;
+    for (int lo = 0; lo + w2 <= n; ++lo) {
+      std::vector<size_t> cur;
+      std::vector<char> blocked(n + 1, 0);
+      for (size_t r = 0; r < steps.size(); ++r) {
+        if (done[r]) continue;
+        const int p = steps[r].p;
+        if (p >= lo && p + 1 < lo + w2 && !blocked[p] && !blocked[p + 1])
+          cur.push_back(r);
+        else
+          blocked[p] = blocked[p + 1] = 1;
+      }
+      if (cur.size() > best.size()) {
+        best = cur;
+        best_lo = lo;
+      }
+    }
+    if (best.empty()) throw std::runtime_error("sub-pass planner made no progress");
+    PlannedPass P;
+    P.lo = best_lo;
+    for (size_t r : best) {
+      KStep k = steps[r];
+      k.p -= best_lo;
+      P.steps.push_back(k);
+      done[r] = 1;
+      ++n_done;
+    }
+    out.push_back(P);
+  }
+  return out;
+}
+
 void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
                      std::vector<SubPass>& subs, std::vector<LayerPair>& layers) {
   subs.clear();
   layers.clear();
   if (window_steps.empty()) return;
   w2 = std::min(w2, w);
+  // zigzag over evenly spaced sub-windows or greedy best start: keep the
+  // schedule with fewer sub-passes
   std::vector<PlannedPass> sp = plan_passes(window_steps, w, w2, false, -1);
+  {
+    std::vector<PlannedPass> alt = plan_greedy_max(window_steps, w, w2);
+    if (alt.size() < sp.size()) sp.swap(alt);
+  }
   for (const PlannedPass& P : sp) {
     if (P.steps.empty()) continue;
     // ASAP layering with layer parity fixed by the pair position

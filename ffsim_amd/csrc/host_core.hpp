@@ -78,6 +78,18 @@ struct GroupTables {
 };
 GroupTables make_group_tables(int w, int w2);
 
+// Static wave schedule of the register groups: for every (kp, lo2) the
+// groups are cut into chunks of at most `per_chunk` consecutive groups with
+// the same k2 (one chunk fills a wave: per_chunk groups x W batch columns),
+// and the chunks are assigned to the block's waves longest-first, balancing
+// the rotation work per SIMD (wave i runs on SIMD i % 4).
+struct WaveSchedule {
+  int n_waves = 0;
+  std::vector<uint32_t> chunks;  // first_group << 8 | n_groups
+  std::vector<int32_t> index;    // [((kp * n_lo2 + lo2) * n_waves + wave) * 2 + {offset, count}]
+};
+WaveSchedule make_wave_schedule(const GroupTables& T, int n_waves, int per_chunk);
+
 // ------------------------------------------------------ Givens decomposition
 // Exact Clements decomposition U = D G_L^* ... G_1^* with each G acting on
 // adjacent orbitals (i, j) as [[c, s], [-conj(s), c]] (c real), matching the

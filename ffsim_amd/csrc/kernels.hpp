@@ -157,6 +157,8 @@ struct GroupArgs {
   const uint16_t* members;
   const int32_t* index;
   int n_lo2, w2;
+  const uint32_t* chunks;   // wave schedule (see WaveSchedule)
+  const int32_t* wave_idx;
 };
 
 // Distinct sub-window starts of a pass (a SubPass's `slot` indexes these).
@@ -214,7 +216,12 @@ __global__ void __launch_bounds__(NTHR)
   uint32_t* gdesc = reinterpret_cast<uint32_t*>(smem_raw + off);
   off += align16((size_t)gd_cap * sizeof(uint32_t));
   uint16_t* gmem = reinterpret_cast<uint16_t*>(smem_raw + off);
-  (void)gm_cap;
+  off += align16((size_t)gm_cap * sizeof(uint16_t));
+  // wave schedule of the staged fiber: per slot and wave (chunk offset, count), then chunks
+  constexpr int NW = NTHR / 64;
+  int32_t* wsched = reinterpret_cast<int32_t*>(smem_raw + off);
+  off += (size_t)kMaxLoSlots * NW * 2 * sizeof(int32_t);
+  uint32_t* wchunks = reinterpret_cast<uint32_t*>(smem_raw + off);
 
   const int tid = threadIdx.x;
   const int n_bt = (n_batch + W - 1) / W;
@@ -239,6 +246,20 @@ __global__ void __launch_bounds__(NTHR)
         for (int k = tid; k < nm; k += NTHR) gmem[m_base + k] = ga.members[m0 + k];
         d_base += nd;
         m_base += nm;
+      }
+      // wave schedule: the chunks of one (kp, lo2) are contiguous over its waves
+      int c_base = 0;
+      for (int u = 0; u < los.n; ++u) {
+        const int32_t* wi = ga.wave_idx + ((fd.kp * ga.n_lo2 + los.lo2[u]) * NW) * 2;
+        const int first = wi[0];
+        int total = 0;
+        for (int wv = 0; wv < NW; ++wv) total += wi[2 * wv + 1];
+        if (tid < NW) {
+          wsched[(u * NW + tid) * 2] = c_base + (wi[2 * tid] - first);
+          wsched[(u * NW + tid) * 2 + 1] = wi[2 * tid + 1];
+        }
+        for (int k = tid; k < total; k += NTHR) wchunks[c_base + k] = ga.chunks[first + k];
+        c_base += total;
       }
     }
   };
@@ -369,15 +390,22 @@ __global__ void __launch_bounds__(NTHR)
     if (prefetch) load_regs(buf, nr, b0n, nbn);
 
     // ---- register sub-passes on the current tile
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int lane = tid % 64;
     for (int sp = 0; sp < n_sub; ++sp) {
       __syncthreads();
       const SubPass P = subs[sp];
       const GroupSlot gs = gslot[P.slot];
       const LayerPair* lp = layers + P.lp0;
-      for (int item = tid; item < gs.nd * W; item += NTHR) {
-        const int g = item / W, j = item % W;
-        const uint32_t d = gdesc[gs.d_off + g];
-        group_dispatch<T, W2>(d & 0xff, tile, S, j, gmem + (d >> 8), lp, P.n_lp);
+      const int c0 = wsched[(P.slot * NW + wave) * 2];
+      const int nc = wsched[(P.slot * NW + wave) * 2 + 1];
+      for (int c = 0; c < nc; ++c) {
+        const uint32_t ch = wchunks[c0 + c];
+        const int gl = lane / W, j = lane % W;
+        if (gl < (int)(ch & 0xff)) {
+          const uint32_t d = gdesc[gs.d_off + (int)(ch >> 8) + gl];
+          group_dispatch<T, W2>(d & 0xff, tile, S, j, gmem + (d >> 8), lp, P.n_lp);
+        }
       }
     }
     __syncthreads();

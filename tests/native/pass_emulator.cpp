@@ -1,0 +1,73 @@
+// CPU emulation of the fused Givens passes (test infrastructure).
+//
+// Runs the host planner (window passes, register sub-passes, layer pairs),
+// the window fiber tables and the register group tables exactly as the GPU
+// fiber kernel consumes them, but with plain loops, so that the host side of
+// the orbital rotation can be checked against the oracle without a GPU.
+// Built by tests/test_host_planner.py with g++.
+#include "../../ffsim_amd/csrc/host_core.hpp"
+#include <cstring>
+#include <complex>
+using namespace ffsim_amd;
+typedef std::complex<double> C;
+
+static std::vector<std::pair<int,int>> sub_pairs(int w2, int k2, int p) {
+  std::vector<std::pair<int,int>> out; auto strs = make_strings(w2, k2);
+  for (size_t r = 0; r < strs.size(); ++r) { uint64_t s = strs[r];
+    if (((s >> p) & 1) && !((s >> (p + 1)) & 1)) out.push_back({(int)r, (int)string_rank(s ^ (3ull << p))}); }
+  return out;
+}
+static void run_pass(const C* src, C* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
+                     const WindowLayout& L, bool all, const GroupTables& GT, const std::vector<KStep>& steps,
+                     const std::vector<C>* diag, const std::vector<uint64_t>& strings) {
+  std::vector<SubPass> subs; std::vector<LayerPair> lps;
+  plan_sub_passes(steps, L.w, kMaxSubWindow, subs, lps);
+  int nf = all ? (int)L.fibers.size() : L.n_active;
+  for (int f = 0; f < nf; ++f) {
+    const Fiber& fd = L.fibers[f];
+    const int32_t* rows = L.row_table.data() + fd.row_off;
+    for (int b = 0; b < n_batch; ++b) {
+      std::vector<C> v(fd.n_rows);
+      for (int i = 0; i < fd.n_rows; ++i) v[i] = src[(long)rows[i] * ld_src + b];
+      for (auto& sp : subs) {
+        int gi = (fd.kp * GT.n_lo2 + sp.lo2) * 4; int goff = GT.index[gi], ng = GT.index[gi + 1];
+        for (int g = 0; g < ng; ++g) {
+          uint32_t d = GT.desc[goff + g]; int k2 = d & 0xff; const uint16_t* mem = GT.members.data() + (d >> 8);
+          int N = binom(GT.w2, k2); std::vector<C> r(N);
+          for (int m = 0; m < N; ++m) r[m] = v[mem[m]];
+          for (int l = 0; l < sp.n_lp; ++l) { const LayerPair& LP = lps[sp.lp0 + l];
+            for (int p : {0, 2, 4, 1, 3}) { if (p + 1 >= GT.w2 || !(LP.mask & (1 << p))) continue;
+              C sg(LP.sr[p], LP.si[p]);
+              for (auto& xy : sub_pairs(GT.w2, k2, p)) { C a = r[xy.first], bb = r[xy.second];
+                r[xy.first] = LP.c[p] * a + sg * bb; r[xy.second] = LP.c[p] * bb - std::conj(sg) * a; } } }
+          for (int m = 0; m < N; ++m) v[mem[m]] = r[m];
+        }
+      }
+      for (int i = 0; i < fd.n_rows; ++i) {
+        C val = v[i];
+        if (diag) { uint64_t bits = strings[rows[i]]; for (int o = 0; o < 64; ++o) if ((bits >> o) & 1) val *= (*diag)[o]; }
+        if (tstore) dst[(long)b * ld_dst + rows[i]] = val; else dst[(long)rows[i] * ld_dst + b] = val;
+      }
+    }
+  }
+}
+
+static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* mat, int w, C* tdst, long ld_t) {
+  std::vector<KStep> steps; std::vector<C> diag(norb, 1.0); bool has_diag = false;
+  if (mat) { auto g = givens_decomposition(mat, norb, 1e-12); for (auto& r : g.rotations) steps.push_back(kernel_step_from_rotation(r)); diag = g.diag; has_diag = true; }
+  auto passes = plan_passes(steps, norb, w, tdst != nullptr, -1);
+  auto strings = make_strings(norb, nocc);
+  auto GT = make_group_tables(w, kMaxSubWindow);
+  for (auto& P : passes) {
+    auto L = make_window_layout(norb, nocc, P.lo, w);
+    bool ts = P.last && tdst; bool dg = P.last && has_diag;
+    run_pass(state, ts ? tdst : state, ld, ts ? ld_t : ld, n_batch, ts, L, ts || dg, GT, P.steps, dg ? &diag : nullptr, strings);
+  }
+}
+
+extern "C" void emulate(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w) {
+  long da = binom(norb, na), db = binom(norb, nb);
+  C* v = (C*)vec; C* wk = (C*)work;
+  sector(v, db, (int)db, norb, na, (const C*)ma, w, wk, da);      // alpha rows, tstore into work
+  sector(wk, da, (int)da, norb, nb, (const C*)mb, w, v, db);      // beta rows on transposed, tstore back
+}

@@ -15,6 +15,7 @@ from ffsim_amd.gates import (
 from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedHamiltonian
 from ffsim_amd.protocols import apply_unitary, linear_operator
 from ffsim_amd.states import dim, dims, hartree_fock_state
+from ffsim_amd.trotter import simulate_trotter_double_factorized
 from ffsim_amd.variational import UCJOpSpinBalanced
 
 __all__ = [
@@ -32,4 +33,5 @@ __all__ = [
     "linalg",
     "linear_operator",
     "random",
+    "simulate_trotter_double_factorized",
 ]

@@ -731,6 +731,33 @@ void check_sizes(int norb, int na, int nb) {
   require(na >= 0 && nb >= 0, FFSIM_ERR_VALUE, "electron counts must be non-negative");
 }
 
+// Sequence of gates  V_{n}^... : for t = 0..n_terms-1: rotate by rot[t], then
+// diag-Coulomb evolution by mats[t] for time times[t]; finally rotate by
+// rot[n_terms].  rot[t] == nullptr skips that rotation.
+void diag_coulomb_sequence_impl(Context& ctx, void* vec, void* work, int dtype, int norb, int na,
+                                int nb, int n_terms, const cdouble* const* rots,
+                                const double* const* mats /* aa, ab, bb per term (3 ptrs) */,
+                                const double* times, int z_rep, hipStream_t st) {
+  for (int t = 0; t < n_terms; ++t) {
+    if (rots[t]) orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[t], rots[t], st);
+    DiagSpec s = diag_coulomb_spec(norb, mats[3 * t], mats[3 * t + 1], mats[3 * t + 2], z_rep,
+                                   times[t]);
+    if (s.flags) diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, norb, na, nb, s, st);
+  }
+  if (rots[n_terms])
+    orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[n_terms], rots[n_terms], st);
+}
+
+std::vector<cdouble> matmul(const cdouble* A, const cdouble* B, int n, bool adj_a) {
+  std::vector<cdouble> C((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k) {
+      const cdouble a = adj_a ? std::conj(A[(size_t)k * n + i]) : A[(size_t)i * n + k];
+      for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+    }
+  return C;
+}
+
 }  // namespace
 }  // namespace ffsim_amd
 
@@ -958,40 +985,132 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
     Context& ctx = context();
     const int n = norb;
     const size_t nn = (size_t)n * n;
-    const cdouble* rots = reinterpret_cast<const cdouble*>(orbital_rotations);
-    hipStream_t st = (hipStream_t)stream;
-    // current_basis starts as the identity (ucj_spin_balanced.py:663)
-    std::vector<cdouble> current(nn, 0.0), m(nn);
-    for (int k = 0; k < n; ++k) current[k * n + k] = 1.0;
+    const cdouble* U = reinterpret_cast<const cdouble*>(orbital_rotations);
+    // V_r = U_r^H U_{r-1} (U_{-1} = I), V_final = F U_last or U_last
+    // (python/ffsim/variational/ucj_spin_balanced.py:657-696)
+    std::vector<std::vector<cdouble>> V(n_reps + 1);
+    std::vector<cdouble> eye(nn, 0.0);
+    for (int k = 0; k < n; ++k) eye[(size_t)k * n + k] = 1.0;
+    for (int r = 0; r < n_reps; ++r)
+      V[r] = matmul(U + r * nn, r == 0 ? eye.data() : U + (r - 1) * nn, n, true);
+    const cdouble* last = n_reps ? U + (n_reps - 1) * nn : eye.data();
+    if (final_orbital_rotation)
+      V[n_reps] = matmul(reinterpret_cast<const cdouble*>(final_orbital_rotation), last, n, false);
+    else
+      V[n_reps] = std::vector<cdouble>(last, last + nn);
+    std::vector<const cdouble*> rots(n_reps + 1);
+    for (int r = 0; r <= n_reps; ++r) rots[r] = V[r].data();
+    std::vector<std::vector<double>> halved(n_reps);
+    std::vector<const double*> mats(3 * n_reps);
+    std::vector<double> times(n_reps, -1.0);
     for (int r = 0; r < n_reps; ++r) {
-      const cdouble* U = rots + r * nn;
-      // m = U^H @ current
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-          cdouble acc = 0.0;
-          for (int k = 0; k < n; ++k) acc += std::conj(U[k * n + i]) * current[k * n + j];
-          m[i * n + j] = acc;
-        }
-      orbital_rotation_impl(ctx, vec, work, dtype, n, n_alpha, n_beta, m.data(), m.data(), st);
       const double* J = diag_coulomb_mats + r * 2 * nn;
-      std::vector<double> aa(J, J + nn), ab(J + nn, J + 2 * nn);
-      for (int k = 0; k < n; ++k) aa[k * n + k] *= 0.5;
-      DiagSpec s = diag_coulomb_spec(n, aa.data(), ab.data(), aa.data(), 0, -1.0);
-      if (s.flags) diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, n, n_alpha, n_beta, s, st);
-      std::copy(U, U + nn, current.begin());
+      halved[r].assign(J, J + nn);
+      for (int k = 0; k < n; ++k) halved[r][(size_t)k * n + k] *= 0.5;
+      mats[3 * r] = halved[r].data();
+      mats[3 * r + 1] = J + nn;
+      mats[3 * r + 2] = halved[r].data();
     }
-    if (final_orbital_rotation) {
-      const cdouble* F = reinterpret_cast<const cdouble*>(final_orbital_rotation);
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-          cdouble acc = 0.0;
-          for (int k = 0; k < n; ++k) acc += F[i * n + k] * current[k * n + j];
-          m[i * n + j] = acc;
-        }
+    diag_coulomb_sequence_impl(ctx, vec, work, dtype, n, n_alpha, n_beta, n_reps, rots.data(),
+                               mats.data(), times.data(), 0, (hipStream_t)stream);
+  });
+}
+
+int ffsim_diag_coulomb_grad_reduce(const void* psi, const void* lam, int dtype, int norb,
+                                   int n_alpha, int n_beta, double* out, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(norb <= 32, FFSIM_ERR_VALUE, "gradient reductions support norb <= 32");
+    Context& ctx = context();
+    const long da = (long)binom(norb, n_alpha), db = (long)binom(norb, n_beta);
+    if (da == 0 || db == 0) return;
+    const DevStrings& cs = dev_strings(ctx, norb, n_beta);
+    hipStream_t st = (hipStream_t)stream;
+    double* rsum = out;
+    double* csum = out + da;
+    double* M = out + da + db;
+    if (dtype == FFSIM_C128) {
+      hipLaunchKernelGGL(grad_rows_kernel<double>, dim3((unsigned)da), dim3(256), 0, st,
+                         (const Cx<double>*)psi, (const Cx<double>*)lam, (int)da, (int)db, norb,
+                         cs.d, rsum, M);
+      hipLaunchKernelGGL(grad_cols_kernel<double>, dim3((unsigned)((db + 255) / 256)), dim3(256),
+                         0, st, (const Cx<double>*)psi, (const Cx<double>*)lam, (int)da, (int)db,
+                         csum);
     } else {
-      m = current;
+      hipLaunchKernelGGL(grad_rows_kernel<float>, dim3((unsigned)da), dim3(256), 0, st,
+                         (const Cx<float>*)psi, (const Cx<float>*)lam, (int)da, (int)db, norb,
+                         cs.d, rsum, M);
+      hipLaunchKernelGGL(grad_cols_kernel<float>, dim3((unsigned)((db + 255) / 256)), dim3(256),
+                         0, st, (const Cx<float>*)psi, (const Cx<float>*)lam, (int)da, (int)db,
+                         csum);
     }
-    orbital_rotation_impl(ctx, vec, work, dtype, n, n_alpha, n_beta, m.data(), m.data(), st);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, int n_alpha,
+                         int n_beta, double* out, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    Context& ctx = context();
+    const long da = (long)binom(norb, n_alpha), db = (long)binom(norb, n_beta);
+    hipStream_t st = (hipStream_t)stream;
+    const long n_out = 4l * norb * norb;
+    if (n_out == 0) return;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<long>((n_out + 255) / 256, 1024)),
+                       dim3(256), 0, st, out, n_out);
+    if (da == 0 || db == 0) return;
+    // binomial table C(c, t) for c, t <= norb, uploaded through the program ring
+    Program prog(ctx, st);
+    std::vector<uint64_t> tab((size_t)(norb + 1) * (norb + 1));
+    for (int c = 0; c <= norb; ++c)
+      for (int t = 0; t <= norb; ++t) tab[(size_t)c * (norb + 1) + t] = binom(c, t);
+    const size_t toff = prog.append(tab.data(), tab.size() * sizeof(uint64_t));
+    prog.commit();
+    const DevStrings& sa = dev_strings(ctx, norb, n_alpha);
+    const DevStrings& sb = dev_strings(ctx, norb, n_beta);
+    double* rdm_a = out;
+    double* rdm_b = out + 2l * norb * norb;
+    if (dtype == FFSIM_C128) {
+      hipLaunchKernelGGL(trdm_kernel<double>, dim3((unsigned)da), dim3(256), 0, st,
+                         (const Cx<double>*)lam, (const Cx<double>*)psi, (int)da, (int)db, norb, 0,
+                         sa.d, prog.dev<uint64_t>(toff), rdm_a);
+      hipLaunchKernelGGL(trdm_kernel<double>, dim3((unsigned)db), dim3(256), 0, st,
+                         (const Cx<double>*)lam, (const Cx<double>*)psi, (int)da, (int)db, norb, 1,
+                         sb.d, prog.dev<uint64_t>(toff), rdm_b);
+    } else {
+      hipLaunchKernelGGL(trdm_kernel<float>, dim3((unsigned)da), dim3(256), 0, st,
+                         (const Cx<float>*)lam, (const Cx<float>*)psi, (int)da, (int)db, norb, 0,
+                         sa.d, prog.dev<uint64_t>(toff), rdm_a);
+      hipLaunchKernelGGL(trdm_kernel<float>, dim3((unsigned)db), dim3(256), 0, st,
+                         (const Cx<float>*)lam, (const Cx<float>*)psi, (int)da, (int)db, norb, 1,
+                         sb.d, prog.dev<uint64_t>(toff), rdm_b);
+    }
+    HIP_CHECK(hipGetLastError());
+    prog.finish();
+  });
+}
+
+int ffsim_apply_diag_coulomb_sequence(void* vec, void* work, int dtype, int norb, int n_alpha,
+                                      int n_beta, int n_terms, const double* rotations,
+                                      const double* mats, const double* times,
+                                      int z_representation, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(n_terms >= 0, FFSIM_ERR_VALUE, "n_terms must be non-negative");
+    Context& ctx = context();
+    const size_t nn = (size_t)norb * norb;
+    std::vector<const cdouble*> rots(n_terms + 1);
+    for (int t = 0; t <= n_terms; ++t)
+      rots[t] = reinterpret_cast<const cdouble*>(rotations) + t * nn;
+    std::vector<const double*> m(3 * n_terms);
+    for (int t = 0; t < n_terms; ++t)
+      for (int k = 0; k < 3; ++k) m[3 * t + k] = mats + (3 * t + k) * nn;
+    diag_coulomb_sequence_impl(ctx, vec, work, dtype, norb, n_alpha, n_beta, n_terms, rots.data(),
+                               m.data(), times, z_representation, (hipStream_t)stream);
   });
 }
 

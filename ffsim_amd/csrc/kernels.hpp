@@ -563,6 +563,142 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (gradient) reductions.
+//
+// grad_rows_kernel: for one row a per block, with w(a,b) = Im(conj(lam) psi):
+//   rsum[a] = sum_b w,   M[a][j] = sum_{b : j in b} w   (j < norb <= 32).
+// grad_cols_kernel: csum[b] = sum_a w (one column per thread).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    grad_rows_kernel(const Cx<T>* __restrict__ psi, const Cx<T>* __restrict__ lam, int n_rows,
+                     int n_cols, int norb, const uint64_t* __restrict__ col_str,
+                     double* __restrict__ rsum, double* __restrict__ M) {
+  __shared__ double acc[32 * 256];
+  __shared__ double red[256];
+  const int a = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int j = 0; j < norb; ++j) acc[j * 256 + tid] = 0.0;
+  double rs = 0.0;
+  for (int c = tid; c < n_cols; c += 256) {
+    const Cx<T> p = psi[(long)a * n_cols + c], l = lam[(long)a * n_cols + c];
+    const double w = (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+    rs += w;
+    uint64_t bits = col_str[c];
+    while (bits) {
+      const int j = __builtin_ctzll(bits);
+      bits &= bits - 1;
+      acc[j * 256 + tid] += w;
+    }
+  }
+  red[tid] = rs;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      red[tid] += red[tid + s];
+      for (int j = 0; j < norb; ++j) acc[j * 256 + tid] += acc[j * 256 + tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    rsum[a] = red[0];
+    for (int j = 0; j < norb; ++j) M[(long)a * norb + j] = acc[j * 256];
+  }
+}
+
+template <typename T>
+__global__ void grad_cols_kernel(const Cx<T>* __restrict__ psi, const Cx<T>* __restrict__ lam,
+                                 int n_rows, int n_cols, double* __restrict__ csum) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cols) return;
+  double s = 0.0;
+  for (int a = 0; a < n_rows; ++a) {
+    const Cx<T> p = psi[(long)a * n_cols + c], l = lam[(long)a * n_cols + c];
+    s += (double)l.re * (double)p.im - (double)l.im * (double)p.re;
+  }
+  csum[c] = s;
+}
+
+// Transition one-body RDM of one spin sector:
+//   rdm[p][q] += sum over strings S with q in S and (p == q or p not in S) of
+//   sign(a+_p a_q |S>) * sum_other conj(lam[S']) psi[S],   S' = S - q + p.
+// sector 0: strings index rows (dot products along rows);
+// sector 1: strings index columns (dot products along columns).
+// The sign is (-1)^(occupied below q in S) * (-1)^(occupied below p in S - q).
+__device__ __forceinline__ uint64_t colex_rank_dev(uint64_t s, const uint64_t* binom_tab,
+                                                   int stride) {
+  uint64_t r = 0;
+  int t = 0;
+  while (s) {
+    const int c = __builtin_ctzll(s);
+    s &= s - 1;
+    ++t;
+    if (c >= t) r += binom_tab[c * stride + t];
+  }
+  return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    trdm_kernel(const Cx<T>* __restrict__ lam, const Cx<T>* __restrict__ psi, int n_rows,
+                int n_cols, int norb, int sector, const uint64_t* __restrict__ strings,
+                const uint64_t* __restrict__ binom_tab, double* __restrict__ rdm) {
+  __shared__ double red_re[256], red_im[256];
+  const int s_idx = blockIdx.x;  // string of this sector
+  const int tid = threadIdx.x;
+  const uint64_t S = strings[s_idx];
+  const int n_other = (sector == 0) ? n_cols : n_rows;
+  const int stride = norb + 1;
+  uint64_t qbits = S;
+  while (qbits) {
+    const int q = __builtin_ctzll(qbits);
+    qbits &= qbits - 1;
+    const uint64_t S0 = S & ~(1ull << q);
+    const int sign_q = __popcll(S & ((1ull << q) - 1)) & 1;
+    for (int p = 0; p < norb; ++p) {
+      if (p != q && ((S >> p) & 1)) continue;
+      const uint64_t S1 = S0 | (1ull << p);
+      const int sign_p = __popcll(S0 & ((1ull << p) - 1)) & 1;
+      const double sgn = ((sign_q ^ sign_p) & 1) ? -1.0 : 1.0;
+      const long t_idx = (long)colex_rank_dev(S1, binom_tab, stride);
+      double re = 0.0, im = 0.0;
+      for (int o = tid; o < n_other; o += 256) {
+        Cx<T> l, x;
+        if (sector == 0) {
+          l = lam[t_idx * n_cols + o];
+          x = psi[(long)s_idx * n_cols + o];
+        } else {
+          l = lam[(long)o * n_cols + t_idx];
+          x = psi[(long)o * n_cols + s_idx];
+        }
+        // conj(l) * x
+        re += (double)l.re * (double)x.re + (double)l.im * (double)x.im;
+        im += (double)l.re * (double)x.im - (double)l.im * (double)x.re;
+      }
+      red_re[tid] = re;
+      red_im[tid] = im;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+          red_re[tid] += red_re[tid + s];
+          red_im[tid] += red_im[tid + s];
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
+        atomicAdd(&rdm[2 * (p * norb + q)], sgn * red_re[0]);
+        atomicAdd(&rdm[2 * (p * norb + q) + 1], sgn * red_im[0]);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void fill_zero_kernel(double* __restrict__ x, long n) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+    x[k] = 0.0;
+}
+
 // out = alpha x + beta y over n complex entries.
 template <typename T>
 __global__ void axpby_kernel(Cx<T>* out, const Cx<T>* x,

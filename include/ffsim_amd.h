@@ -138,6 +138,36 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
                                   const double* orbital_rotations,
                                   const double* final_orbital_rotation, void* stream);
 
+/* ---- adjoint reductions (differentiation; no counterpart in the reference,
+ * whose only gradient is finite differences, python/ffsim/optimize/_util.py:60-110) */
+
+/* Phase backward of a diag-Coulomb gate: with w(a,b) = Im(conj(lam[a,b]) psi[a,b]),
+ * writes into the DEVICE buffer out (dim_a + dim_b + dim_a*norb doubles):
+ * row sums (dim_a), column sums (dim_b) and M[a][j] = sum over b with orbital j
+ * occupied of w(a,b).  norb <= 32. */
+int ffsim_diag_coulomb_grad_reduce(const void* psi, const void* lam, int dtype, int norb,
+                                   int n_alpha, int n_beta, double* out, void* stream);
+
+/* Transition one-body RDM rdm_sigma[p][q] = <lam| a+_{p sigma} a_{q sigma} |psi> for
+ * sigma = alpha then beta, as 2 x norb x norb complex128 in the DEVICE buffer out
+ * (zeroed by the call). */
+int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, int n_alpha,
+                         int n_beta, double* out, void* stream);
+
+/* A sequence of rotated diagonal-Coulomb evolutions, the gate pattern shared
+ * by UCJOpSpinBalanced._apply_unitary_ (ucj_spin_balanced.py:657-696) and the
+ * double-factorized Trotter step (python/ffsim/trotter/double_factorized.py:
+ * 89-127): for t in 0..n_terms-1, rotate both spin sectors by
+ * rotations[t] then evolve by the diag-Coulomb matrices mats[t] = (aa, ab, bb)
+ * for times[t]; finally rotate by rotations[n_terms].
+ * rotations: (n_terms + 1) x norb x norb complex128; mats: n_terms x 3 x norb x
+ * norb real, prepared as for ffsim_apply_diag_coulomb_evolution (number
+ * representation: aa/bb diagonals halved); times: n_terms. */
+int ffsim_apply_diag_coulomb_sequence(void* vec, void* work, int dtype, int norb, int n_alpha,
+                                      int n_beta, int n_terms, const double* rotations,
+                                      const double* mats, const double* times,
+                                      int z_representation, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

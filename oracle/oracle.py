@@ -601,6 +601,52 @@ def apply_ucj_spin_balanced(vec, diag_coulomb_mats, orbital_rotations, final_orb
     return apply_orbital_rotation(vec, last, norb, nelec, copy=False)
 
 
+def trotter_step_iterator(n_terms, time, order):
+    """python/ffsim/trotter/_util.py:18-55."""
+    if order == 0:
+        for i in range(n_terms):
+            yield i, time
+        return
+    if order == 1:
+        for i in range(n_terms - 1):
+            yield i, time / 2
+        yield n_terms - 1, time
+        for i in reversed(range(n_terms - 1)):
+            yield i, time / 2
+        return
+    split = time / (4 - 4 ** (1 / (2 * order - 1)))
+    for _ in range(2):
+        yield from trotter_step_iterator(n_terms, split, order - 1)
+    yield from trotter_step_iterator(n_terms, time - 4 * split, order - 1)
+    for _ in range(2):
+        yield from trotter_step_iterator(n_terms, split, order - 1)
+
+
+def simulate_trotter_double_factorized(vec, one_body_tensor, diag_coulomb_mats, orbital_rotations,
+                                       constant, time, norb, nelec, n_steps=1, order=0,
+                                       z_representation=False):
+    """python/ffsim/trotter/double_factorized.py:25-127."""
+    vec = np.asarray(vec, dtype=np.complex128).copy()
+    if n_steps == 0:
+        return vec
+    step_time = time / n_steps
+    current = np.eye(norb, dtype=complex)
+    for _ in range(n_steps):
+        for term, t in trotter_step_iterator(1 + len(diag_coulomb_mats), step_time, order):
+            if term == 0:
+                current = scipy.linalg.expm(-1j * t * one_body_tensor) @ current
+            else:
+                U = orbital_rotations[term - 1]
+                vec = apply_orbital_rotation(vec, U.T.conj() @ current, norb, nelec, copy=False)
+                vec = apply_diag_coulomb_evolution(vec, diag_coulomb_mats[term - 1], t, norb, nelec,
+                                                   z_representation=z_representation, copy=False)
+                current = U
+    vec = apply_orbital_rotation(vec, current, norb, nelec, copy=False)
+    if constant:
+        vec = vec * np.exp(-1j * time * constant)
+    return vec
+
+
 # ---------------------------------------------------------------------------
 # Layer 2: independent closed forms
 # ---------------------------------------------------------------------------

@@ -298,3 +298,20 @@ def test_complex64_error_vs_fp64_oracle(norb, nelec, rng):
     print(f"c64 norb={norb} |d|inf={err:.3e} 1-|<ref|psi>|={infidelity:.3e}", flush=True)
     assert err < 5e-5
     assert infidelity < 1e-6
+
+
+@pytest.mark.parametrize("order,n_steps,z_rep", [(0, 2, False), (1, 2, False), (2, 1, True)])
+def test_trotter_double_factorized(order, n_steps, z_rep, rng):
+    import ffsim_amd
+
+    norb, nelec = 8, (4, 3)
+    _case("trotter", order, n_steps, z_rep)
+    df = ffsim_amd.random.random_double_factorized_hamiltonian(norb, rank=5,
+                                                               z_representation=z_rep, seed=rng)
+    vec = _rand(norb, nelec, rng)
+    expected = O.simulate_trotter_double_factorized(
+        vec, df.one_body_tensor, df.diag_coulomb_mats, df.orbital_rotations, df.constant, 0.3,
+        norb, nelec, n_steps=n_steps, order=order, z_representation=z_rep)
+    got = ffsim_amd.simulate_trotter_double_factorized(vec, df, 0.3, norb=norb, nelec=nelec,
+                                                       n_steps=n_steps, order=order)
+    np.testing.assert_allclose(got, expected, atol=1e-11, rtol=0)

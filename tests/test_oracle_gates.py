@@ -268,3 +268,36 @@ def test_planted_mistake_in_diag_halving_is_caught(rng, monkeypatch):
     E = O.diag_coulomb_energies(J, norb, nelec)
     assert np.abs(bad - np.exp(-0.7j * E) * vec).max() > 1e-3
     monkeypatch.setattr(O, "get_mat_exp", real)
+
+
+@pytest.mark.parametrize("z_rep", [False, True])
+def test_trotter_vs_exact_propagator(z_rep, rng):
+    """trotter/double_factorized_test.py:23-83 restated: Trotter converges to
+    expm(-i t H) of the dense H_DF; order 1/2 errors shrink as expected."""
+    norb, nelec = 4, (2, 1)
+    h = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    h = 0.1 * (h + h.conj().T)
+    mats = [0.1 * O.random_symmetric_uniform(norb, rng) for _ in range(3)]
+    rots = [O.random_unitary(norb, rng) for _ in range(3)]
+    vec = _vec(norb, nelec, rng)
+    t = 0.5
+    exact = scipy.linalg.expm(-1j * t * _dense_df(h, mats, rots, 0.2, norb, nelec, z_rep)) @ vec
+    errs = []
+    for order, n_steps in ((0, 10), (1, 10), (2, 3)):
+        got = O.simulate_trotter_double_factorized(vec, h, mats, rots, 0.2, t, norb, nelec,
+                                                   n_steps=n_steps, order=order,
+                                                   z_representation=z_rep)
+        errs.append(np.linalg.norm(got - exact))
+    assert errs[0] < 3e-2 and errs[1] < 3e-3 and errs[2] < 3e-4, errs
+
+
+def test_trotter_single_term_is_exact(rng):
+    """One diag-Coulomb factor and no one-body term: the formula is exact."""
+    norb, nelec = 4, (2, 2)
+    J = O.random_symmetric_uniform(norb, rng)
+    U = O.random_unitary(norb, rng)
+    vec = _vec(norb, nelec, rng)
+    got = O.simulate_trotter_double_factorized(vec, np.zeros((norb, norb)), [J], [U], 0.0, 0.9,
+                                               norb, nelec, n_steps=1)
+    exact = scipy.linalg.expm(-0.9j * _dense_df(np.zeros((norb, norb)), [J], [U], 0.0, norb, nelec)) @ vec
+    np.testing.assert_allclose(got, exact, atol=1e-12)

@@ -91,6 +91,98 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> di
     }
 
 
+def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
+    """Served mixed stream vs the unbatched plain path, same Poisson arrivals.
+
+    Requests: LUCJ n_reps=2 (pairs aa=(p,p+1), ab=(p,p)) on a random state,
+    norb uniform in {8, 10, 12, 14}, nelec (n/2, n/2) or (n/2, n/2 - 1),
+    complex128, numpy in and numpy out.  The arrival rate is ``load`` times the
+    plain path's capacity (1 / mean service time measured alone).
+    """
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd.serving import LUCJRequest, LUCJServer, request_footprint
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_requests):
+        norb = int(rng.choice([8, 10, 12, 14]))
+        nelec = (norb // 2, norb // 2 - int(rng.integers(0, 2)))
+        op = ffsim_amd.random.random_ucj_op_spin_balanced(
+            norb, n_reps=2, interaction_pairs=lucj_pairs(norb), seed=rng
+        )
+        dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+        state = ffsim_amd.random.random_state_vector(dim, seed=rng)
+        reqs.append(LUCJRequest(op=op, norb=norb, nelec=nelec, state=state))
+
+    def plain(r):
+        return ffsim_amd.apply_unitary(r.state, r.op, r.norb, r.nelec)
+
+    for r in reqs[:16]:  # warm tables for every shape
+        plain(r)
+    service = []
+    for r in reqs:
+        t0 = time.perf_counter()
+        plain(r)
+        service.append(time.perf_counter() - t0)
+    mean_service = float(np.mean(service))
+    rate = load / mean_service
+    gaps = rng.exponential(1.0 / rate, size=n_requests)
+    offsets = np.cumsum(gaps)
+
+    # unbatched plain path: one request at a time, in arrival order
+    lat_plain = []
+    start = time.perf_counter()
+    for r, off in zip(reqs, offsets):
+        while time.perf_counter() - start < off:
+            pass
+        plain(r)
+        lat_plain.append(time.perf_counter() - start - off)
+    span_plain = time.perf_counter() - start
+
+    # continuous batching
+    budget = 4 * max(request_footprint(r.norb, r.nelec, r.dtype) for r in reqs)
+    server = LUCJServer(hbm_budget=budget, slots_per_stream=2, n_streams=4, poll_interval=0.0)
+    tickets = []
+    start = time.perf_counter()
+    k = 0
+    while k < n_requests or server.step():
+        now = time.perf_counter() - start
+        while k < n_requests and offsets[k] <= now:
+            tickets.append(server.submit(reqs[k], now=start + offsets[k]))
+            k += 1
+        server.step()
+    span_served = time.perf_counter() - start
+    torch.cuda.synchronize()
+    lat_served = [t.latency for t in tickets]
+    mismatches = sum(
+        int(not np.array_equal(t.result(), plain(r))) for t, r in zip(tickets[:32], reqs[:32])
+    )
+
+    def summary(lat, span):
+        lat = np.asarray(lat)
+        return {
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "states_per_s": round(n_requests / span, 1),
+        }
+
+    return {
+        "workload": "LUCJ n_reps=2, norb in {8,10,12,14}, nelec (n/2,n/2) or (n/2,n/2-1), "
+        "c128, numpy in/out, Poisson arrivals",
+        "requests": n_requests,
+        "offered_load_vs_plain_capacity": load,
+        "arrival_rate_per_s": round(rate, 1),
+        "mean_plain_service_ms": round(mean_service * 1e3, 3),
+        "served": summary(lat_served, span_served),
+        "plain": summary(lat_plain, span_plain),
+        "hbm_budget_bytes": budget,
+        "peak_hbm_bytes": server.peak_bytes,
+        "served_vs_alone_bitwise_mismatches_of_32": mismatches,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +192,17 @@ def main() -> int:
     ap.add_argument("--n-reps", type=int, default=4)
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=400)
+    ap.add_argument("--serving-load", type=float, default=0.9)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument(
+        "--copy-probe",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="only run 3 device-to-device copies of MIB MiB each (calibrates HBM byte counters)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -117,6 +219,15 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 copies of {n} bytes (read {n} + write {n} per copy)")
+        return 0
     norb = args.norb
     nelec = (norb // 2, norb // 2)
     da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
@@ -193,6 +304,10 @@ def main() -> int:
             "achieved_GBs": round(diag_bytes / max(diag_ms, 1e-9) / 1e6, 1),
         },
     }
+    if not args.no_serving:
+        log("serving leg")
+        result["serving"] = serving_leg(args.serving_requests, args.serving_load)
+        log(f"serving: {result['serving']}")
     if not args.no_cpu_baseline:
         vec_host = state.to(torch.complex128).cpu().numpy()
         result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, args.cpu_threads)

@@ -166,3 +166,24 @@ def random_double_factorized_hamiltonian(
         constant=constant,
         z_representation=z_representation,
     )
+
+
+def random_df_hamiltonian_normal(
+    norb: int, *, n_factors: int = 8, scale: float = 0.1, seed=None
+) -> DoubleFactorizedHamiltonian:
+    """Synthetic double-factorized Hamiltonian for energy-gradient workloads.
+
+    One-body tensor: real symmetric with N(0, scale) entries; then n_factors
+    Haar orbital rotations; then n_factors real symmetric N(0, scale)
+    diagonal Coulomb matrices (number representation, constant 0).  The
+    rotation-then-matrix draw order follows random.py:911-956; ffsim has no
+    generator with these distributions.
+    """
+    rng = np.random.default_rng(seed)
+    one_body = _symmetric_normal(norb, mean=0.0, scale=scale, seed=rng)
+    rots = np.stack([random_unitary(norb, seed=rng) for _ in range(n_factors)])
+    mats = np.stack([_symmetric_normal(norb, mean=0.0, scale=scale, seed=rng)
+                     for _ in range(n_factors)])
+    return DoubleFactorizedHamiltonian(
+        one_body_tensor=one_body, diag_coulomb_mats=mats, orbital_rotations=rots, constant=0.0
+    )

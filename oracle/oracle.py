@@ -601,6 +601,74 @@ def apply_ucj_spin_balanced(vec, diag_coulomb_mats, orbital_rotations, final_orb
     return apply_orbital_rotation(vec, last, norb, nelec, copy=False)
 
 
+def ucj_energy(params, hamiltonian_matvec, reference_state, norb, nelec, n_reps,
+               interaction_pairs=None, with_final_orbital_rotation=False) -> float:
+    """<psi(theta)|H|psi(theta)> for the spin-balanced UCJ ansatz on the CPU path."""
+    J, U, final = ucj_from_parameters(params, norb, n_reps, interaction_pairs,
+                                      with_final_orbital_rotation)
+    psi = apply_ucj_spin_balanced(reference_state, J, U, final, norb, nelec)
+    return float(np.vdot(psi, hamiltonian_matvec(psi)).real)
+
+
+def gradient_finite_diff(func, theta: np.ndarray, index: int, epsilon: float):
+    """Central difference of one parameter (python/ffsim/optimize/_util.py:60-83)."""
+    unit = np.zeros(len(theta))
+    unit[index] = 1.0
+    return (func(theta + epsilon * unit) - func(theta - epsilon * unit)) / (2 * epsilon)
+
+
+def fd_gradient(func, theta: np.ndarray, epsilon: float = 1e-6, indices=None) -> np.ndarray:
+    """Central finite-difference gradient over ``indices`` (all by default), the
+    reference's only gradient mechanism (python/ffsim/optimize/_util.py:60-110)."""
+    theta = np.asarray(theta, dtype=float)
+    indices = range(len(theta)) if indices is None else indices
+    return np.array([gradient_finite_diff(func, theta, i, epsilon) for i in indices])
+
+
+def transition_rdm(lam, psi, norb, nelec) -> np.ndarray:
+    """rdm[sigma, p, q] = <lam| a+_{p sigma} a_{q sigma} |psi> by string loops.
+
+    Signs follow the Jordan-Wigner ordering of ascending orbitals within a spin
+    (the convention of the orbital-rotation closed form).  Pinned in the tests
+    against the dense one-body matrices and against the derivative of the
+    minors closed form.
+    """
+    na, nb = nelec
+    sa, sb = colex_strings(norb, na), colex_strings(norb, nb)
+    lam = np.asarray(lam).reshape(len(sa), len(sb))
+    psi = np.asarray(psi).reshape(len(sa), len(sb))
+    out = np.zeros((2, norb, norb), dtype=complex)
+    for sigma, strings in enumerate((sa, sb)):
+        index = {s: k for k, s in enumerate(strings)}
+        for k, s in enumerate(strings):
+            for q in range(norb):
+                if not (s >> q) & 1:
+                    continue
+                t0 = s & ~(1 << q)
+                sq = bin(s & ((1 << q) - 1)).count("1")
+                for p in range(norb):
+                    if p != q and (s >> p) & 1:
+                        continue
+                    sign = (-1) ** (sq + bin(t0 & ((1 << p) - 1)).count("1"))
+                    t = index[t0 | (1 << p)]
+                    if sigma == 0:
+                        val = np.vdot(lam[t], psi[k])
+                    else:
+                        val = np.vdot(lam[:, t], psi[:, k])
+                    out[sigma, p, q] += sign * val
+    return out
+
+
+def diag_coulomb_phase_reductions(psi, lam, norb, nelec):
+    """Row sums, column sums and occupation-weighted row sums of
+    w = Im(conj(lam) psi) over the dim_a x dim_b grid."""
+    na, nb = nelec
+    da, db = math.comb(norb, na), math.comb(norb, nb)
+    w = (np.conj(np.asarray(lam)) * np.asarray(psi)).imag.reshape(da, db)
+    occ_b = occupations(colex_strings(norb, nb), norb)
+    return w.sum(axis=1), w.sum(axis=0), w @ occ_b
+
+
 def trotter_step_iterator(n_terms, time, order):
     """python/ffsim/trotter/_util.py:18-55."""
     if order == 0:

@@ -1,0 +1,113 @@
+"""Serving on the GPU: mixed shapes in flight together give the same results
+as each request alone, and the HBM budget holds.
+
+Every test prints the case it runs (flushed) before touching the GPU.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(*args):
+    print("CASE", *args, flush=True)
+    sys.stdout.flush()
+
+
+def _lucj_pairs(norb):
+    return [(p, p + 1) for p in range(norb - 1)], [(p, p) for p in range(norb)]
+
+
+def test_request_footprint_counts_state_and_work_buffer():
+    from ffsim_amd import _lib
+    from ffsim_amd.serving import request_footprint
+
+    _case("footprint")
+    for norb, nelec, dtype in [(8, (4, 4), "c128"), (12, (6, 5), "c64"), (14, (7, 7), "c128")]:
+        elem = 16 if dtype == "c128" else 8
+        n = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+        tag = _lib.C128 if dtype == "c128" else _lib.C64
+        work = _lib.check(_lib.lib().ffsim_orbital_rotation_work_elements(tag, norb, *nelec))
+        fp = request_footprint(norb, nelec, dtype)
+        assert fp >= (n + work) * elem
+        assert fp < (n + work) * elem + 2 * (2 << 20) + 1
+
+
+def _mixed_requests(rng, n, shapes):
+    import ffsim_amd
+    from ffsim_amd.serving import LUCJRequest
+
+    reqs = []
+    for k in range(n):
+        norb, nelec, dtype = shapes[k % len(shapes)]
+        op = ffsim_amd.random.random_ucj_op_spin_balanced(
+            norb, n_reps=2, interaction_pairs=_lucj_pairs(norb), seed=rng
+        )
+        dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+        state = O.random_state_vector(dim, rng)
+        reqs.append(LUCJRequest(op=op, norb=norb, nelec=nelec, state=state, dtype=dtype))
+    return reqs
+
+
+SHAPES = [(8, (4, 4), "c128"), (10, (5, 4), "c128"), (12, (6, 6), "c128"), (8, (4, 3), "c64"),
+          (10, (5, 5), "c64"), (12, (6, 5), "c128")]
+
+
+def test_results_identical_alone_and_in_mix(rng):
+    import torch
+
+    from ffsim_amd.serving import LUCJServer
+
+    _case("serving mix", len(SHAPES))
+    reqs = _mixed_requests(rng, 24, SHAPES)
+    # alone: one request at a time on the default stream
+    alone = []
+    for r in reqs:
+        cdt = torch.complex128 if r.dtype == "c128" else torch.complex64
+        t = torch.from_numpy(r.state).to(cdt).cuda()
+        r.op.apply_in_place(t, r.norb, r.nelec)
+        alone.append(t.cpu().numpy())
+    torch.cuda.synchronize()
+    server = LUCJServer(hbm_budget=64 << 20, slots_per_stream=2, n_streams=4)
+    tickets = [server.submit(r) for r in reqs]
+    server.drain(timeout=120)
+    for r, t, a in zip(reqs, tickets, alone):
+        got = t.result()
+        assert got.dtype == a.dtype
+        np.testing.assert_array_equal(got, a)  # bitwise: deterministic kernels, one stream each
+    # and the c128 results match the oracle
+    for r, t in zip(reqs[:6], tickets[:6]):
+        if r.dtype != "c128":
+            continue
+        expected = O.apply_ucj_spin_balanced(r.state, r.op.diag_coulomb_mats,
+                                             r.op.orbital_rotations, r.op.final_orbital_rotation,
+                                             r.norb, r.nelec)
+        np.testing.assert_allclose(t.result(), expected, atol=1e-12, rtol=0)
+    st = server.stats()
+    print("serving stats", st, flush=True)
+    assert st["served"] == len(reqs) and st["failed"] == 0
+    assert st["peak_hbm_bytes"] <= 64 << 20
+
+
+def test_tight_budget_serializes_large_requests(rng):
+    from ffsim_amd.serving import AdmissionError, LUCJServer, request_footprint
+
+    _case("serving tight budget")
+    big = request_footprint(12, (6, 6), "c128")
+    small = request_footprint(8, (4, 4), "c128")
+    server = LUCJServer(hbm_budget=big + 2 * small, slots_per_stream=2, n_streams=2)
+    reqs = _mixed_requests(rng, 8, [(12, (6, 6), "c128"), (8, (4, 4), "c128")])
+    tickets = [server.submit(r) for r in reqs]
+    server.drain(timeout=120)
+    assert all(t.result() is not None for t in tickets)
+    assert server.stats()["peak_hbm_bytes"] <= big + 2 * small
+    with pytest.raises(AdmissionError):
+        server.submit(_mixed_requests(rng, 1, [(14, (7, 7), "c128")])[0])

@@ -91,6 +91,18 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> di
     }
 
 
+def measured_traffic(kernel: str, workload: str, dtype: str):
+    """HBM bytes per launch of ``kernel`` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    for entry in json.loads(path.read_text()):
+        if (entry.get("kernel"), entry.get("workload"), entry.get("dtype")) == (kernel, workload, dtype):
+            return entry
+    return None
+
+
 def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
     """Served mixed stream vs the unbatched plain path, same Poisson arrivals.
 
@@ -274,10 +286,15 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "traffic_source": None,
         "avg_launch_ms": round(avg_ms, 4),
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "launches_per_step": fiber_n / args.steps,
     }
+    tr = measured_traffic("fiber_kernel", f"lucj_norb{norb}_nreps{args.n_reps}", args.dtype)
+    if tr is not None:
+        roofline["traffic"] = tr["bytes_per_launch"]
+        roofline["traffic_source"] = tr["source"]
     result = {
         "metric": "LUCJ applies/s + Givens HBM GB/s at norb=16 (8,8); served p50/p99 latency",
         "value": round(value, 4),

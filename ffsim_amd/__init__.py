@@ -16,12 +16,14 @@ from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedH
 from ffsim_amd.protocols import apply_unitary, linear_operator
 from ffsim_amd.states import dim, dims, hartree_fock_state
 from ffsim_amd.trotter import simulate_trotter_double_factorized
-from ffsim_amd.variational import UCJOpSpinBalanced
+from ffsim_amd.variational import UCJOpSpinBalanced, UCJOpSpinless, UCJOpSpinUnbalanced
 
 __all__ = [
     "DiagonalCoulombHamiltonian",
     "DoubleFactorizedHamiltonian",
     "UCJOpSpinBalanced",
+    "UCJOpSpinUnbalanced",
+    "UCJOpSpinless",
     "apply_diag_coulomb_evolution",
     "apply_num_op_sum_evolution",
     "apply_orbital_rotation",

@@ -24,24 +24,35 @@ def _dim(norb, nelec):
 
 
 class DeviceLinearOperator(LinearOperator):
-    def __init__(self, norb: int, nelec, matvec_device: Callable[[torch.Tensor], torch.Tensor]):
+    """``rmatvec_device`` defaults to ``matvec_device`` (Hermitian operators)."""
+
+    def __init__(self, norb: int, nelec, matvec_device: Callable[[torch.Tensor], torch.Tensor],
+                 rmatvec_device: Callable[[torch.Tensor], torch.Tensor] | None = None):
         n = _dim(norb, nelec)
         super().__init__(dtype=np.complex128, shape=(n, n))
         self.norb = norb
         self.nelec = nelec
         self.matvec_device = matvec_device
+        self.rmatvec_device = rmatvec_device
+        self.hermitian = rmatvec_device is None
 
-    def _matvec(self, x):
+    def _apply_host(self, fn, x):
         x = np.asarray(x)
         shape = x.shape
         st = _device.to_device(x.reshape(-1), copy=False)
-        out = self.matvec_device(st.tensor)
+        out = fn(st.tensor)
         return st.restore(out).reshape(shape)
 
-    _rmatvec = _matvec
+    def _matvec(self, x):
+        return self._apply_host(self.matvec_device, x)
+
+    def _rmatvec(self, x):
+        return self._apply_host(self.rmatvec_device or self.matvec_device, x)
 
     def _adjoint(self):
-        return self
+        if self.hermitian:
+            return self
+        return DeviceLinearOperator(self.norb, self.nelec, self.rmatvec_device, self.matvec_device)
 
     def matvec(self, x):
         if isinstance(x, torch.Tensor) and x.device.type == "cuda":

@@ -106,6 +106,12 @@ struct DevStrings {
   uint64_t* d = nullptr;
   int64_t dim = 0;
 };
+// One-body link tables of one (norb, nocc): per target string, n_links pairs
+// (source string, p | q << 8 | negative sign << 16).
+struct DevLinks {
+  int32_t* d = nullptr;
+  int n_links = 0;
+};
 struct DevWindow {
   Fiber* fibers = nullptr;
   int32_t* rows = nullptr;
@@ -149,6 +155,7 @@ struct Context {
   int n_cu = 0;
   std::mutex mu;
   std::map<std::pair<int, int>, DevStrings> strings;
+  std::map<std::pair<int, int>, DevLinks> links;
   std::map<std::tuple<int, int, int, int>, DevWindow> windows;
   std::map<int, DevGroups> groups;
   ProgramRing ring;
@@ -239,6 +246,42 @@ const DevStrings& dev_strings(Context& ctx, int norb, int nocc) {
   ds.dim = (int64_t)s.size();
   ds.d = upload_table(s);
   return ctx.strings[key] = ds;
+}
+
+// Links of every string T of (norb, nocc): for p in T and q = p or q not in T,
+// the source S = T - p + q and the sign of a+_p a_q |S> (Jordan-Wigner order of
+// ascending orbitals).  Addresses are colex ranks.
+const DevLinks& dev_links(Context& ctx, int norb, int nocc) {
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    auto it = ctx.links.find({norb, nocc});
+    if (it != ctx.links.end()) return it->second;
+  }
+  std::vector<uint64_t> strs = make_strings(norb, nocc);
+  const int n_links = nocc * (norb - nocc + 1);
+  std::vector<int32_t> tab((size_t)strs.size() * n_links * 2);
+  for (size_t t = 0; t < strs.size(); ++t) {
+    const uint64_t T = strs[t];
+    int l = 0;
+    for (int p = 0; p < norb; ++p) {
+      if (!((T >> p) & 1)) continue;
+      for (int q = 0; q < norb; ++q) {
+        if (q != p && ((T >> q) & 1)) continue;
+        const uint64_t S = (T & ~(1ull << p)) | (1ull << q);
+        const uint64_t S0 = S & ~(1ull << q);
+        const int par = (__builtin_popcountll(S & ((1ull << q) - 1)) +
+                         __builtin_popcountll(S0 & ((1ull << p) - 1))) & 1;
+        tab[(t * n_links + l) * 2] = (int32_t)string_rank(S);
+        tab[(t * n_links + l) * 2 + 1] = p | (q << 8) | (par << 16);
+        ++l;
+      }
+    }
+  }
+  DevLinks dl;
+  dl.n_links = n_links;
+  dl.d = upload_table(tab);
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  return ctx.links.emplace(std::make_pair(norb, nocc), dl).first->second;
 }
 
 const DevWindow& dev_window(Context& ctx, int norb, int nocc, int lo, int w) {
@@ -336,6 +379,8 @@ size_t elem_bytes(int dtype) { return dtype == FFSIM_C64 ? 8 : 16; }
 
 // LDS kept free for the staged program of a pass (sub-passes + steps).
 constexpr size_t kProgramLdsReserve = 20480;
+// LDS kept free beside the tile, row indices and row strings of a window pass.
+constexpr size_t kGroupLdsReserve = 16384;
 
 int tile_width(int dtype) {
   static int env_w = [] {
@@ -343,7 +388,8 @@ int tile_width(int dtype) {
     return e ? std::atoi(e) : 0;
   }();
   if (env_w == 4 || env_w == 8 || env_w == 16) return env_w;
-  return dtype == FFSIM_C64 ? 16 : 8;
+  (void)dtype;
+  return 8;  // 128-byte rows for c128; for c64 8 columns beat 16 (two tiles per CU)
 }
 
 // Largest fiber of a window of width w for nocc electrons on norb orbitals.
@@ -366,10 +412,12 @@ int window_cap() {
 // Largest window (up to the cap) whose fiber tiles fit in LDS.
 int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   const int W = tile_width(dtype);
-  const size_t per_row = (size_t)(W + 1) * elem_bytes(dtype);
+  // tile row + row index + row string (phase passes); the reserve holds the group
+  // tables, wave schedule and orbital phases
+  const size_t per_row = (size_t)(W + 1) * elem_bytes(dtype) + 4 + (norb <= 32 ? 4 : 8);
   int best = std::min(norb, 2);
   for (int w = std::min(norb, window_cap()); w >= 2; --w) {
-    if ((size_t)max_fiber(norb, nocc, w) * per_row + kProgramLdsReserve <= ctx.lds_max) {
+    if ((size_t)max_fiber(norb, nocc, w) * per_row + kGroupLdsReserve <= ctx.lds_max) {
       best = w;
       break;
     }
@@ -378,22 +426,27 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   return best;
 }
 
+// LDS of a phase pass beyond the tile: orbital phases and one string per row.
+size_t diag_lds_bytes(bool diag, int n_diag, int rows) {
+  if (!diag) return 0;
+  return 2 * kMaxDiagOrbitals * sizeof(double) + (size_t)rows * (n_diag <= 32 ? 4 : 8);
+}
+
 template <typename T, int W, int MODE, int W2, int NTHR>
 void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
                     const DevWindow& win, bool all_fibers, const DevGroups& gr, const LoSlots& los,
                     int gd_cap, int gm_cap, const SubPass* d_subs, int n_sub,
                     const LayerPair* d_layers, int n_layers, const double* d_diag,
-                    const uint64_t* d_strings, size_t lds_max, hipStream_t stream) {
+                    const uint64_t* d_strings, int n_diag, size_t lds_max, hipStream_t stream) {
   constexpr int S = (W >= 8) ? W + 1 : W;
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
   if (n_f == 0 || n_batch == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
   const int gc_cap = gd_cap;  // every chunk holds at least one group
   const size_t lds = align16((size_t)rows * S * sizeof(Cx<T>)) + align16((size_t)rows * 4) +
-                     (size_t)n_layers * sizeof(LayerPair) + (size_t)n_sub * sizeof(SubPass) +
                      kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
                      align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * (NTHR / 64) * 8 +
-                     (size_t)gc_cap * 4;
+                     align16((size_t)gc_cap * 4) + diag_lds_bytes(d_diag != nullptr, n_diag, rows);
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
   auto kern = fiber_kernel<T, W, MODE, NTHR, W2>;
   const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
@@ -418,7 +471,8 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
                      ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(sched), los,
-                     d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, rows, gd_cap, gm_cap);
+                     d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap,
+                     gm_cap);
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
 }
@@ -428,21 +482,20 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                   bool tstore, const DevWindow& win, bool all_fibers, const DevGroups& gr,
                   const LoSlots& los, int gd_cap, int gm_cap,
                   const SubPass* d_subs, int n_sub, const LayerPair* d_layers, int n_layers,
-                  const double* d_diag, const uint64_t* d_strings, size_t lds_max, int W,
-                  hipStream_t stream) {
+                  const double* d_diag, const uint64_t* d_strings, int n_diag, size_t lds_max,
+                  int W, hipStream_t stream) {
   // gr.w2 < kMaxSubWindow only for windows narrower than the sub-window
 #define FFSIM_FIBER_LAUNCH_N(WW, MM, W2, NT)                                                    \
   launch_fiber_t<T, WW, MM, W2, NT>(src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers,  \
                                     gr, los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers,  \
-                                    d_diag, d_strings, lds_max, stream)
+                                    d_diag, d_strings, n_diag, lds_max, stream)
   // 512 threads when one block fills the CU's LDS, 256 when two or more fit
   const int rows_l = all_fibers ? win.max_rows : win.max_rows_active;
   const int S_l = (W >= 8) ? W + 1 : W;
   const size_t lds_l = align16((size_t)rows_l * S_l * sizeof(Cx<T>)) + align16((size_t)rows_l * 4) +
-                       (size_t)n_layers * sizeof(LayerPair) + (size_t)n_sub * sizeof(SubPass) +
                        kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
                        align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * 16 * 8 +
-                       (size_t)gd_cap * 4;
+                       align16((size_t)gd_cap * 4) + diag_lds_bytes(d_diag != nullptr, n_diag, rows_l);
   const bool wide = 2 * lds_l > lds_max;
 #define FFSIM_FIBER_LAUNCH(WW, MM, W2)               \
   do {                                               \
@@ -494,16 +547,16 @@ void fiber_pass(int dtype, int mode, const void* src, void* dst, long ld_src, lo
                 int n_batch, bool tstore, const DevWindow& win, bool all_fibers,
                 const DevGroups& gr, const LoSlots& los, int gd_cap, int gm_cap,
                 const SubPass* d_subs, int n_sub, const LayerPair* d_layers,
-                int n_layers, const double* d_diag, const uint64_t* d_strings, Context& ctx,
-                hipStream_t stream) {
+                int n_layers, const double* d_diag, const uint64_t* d_strings, int n_diag,
+                Context& ctx, hipStream_t stream) {
   const int W = tile_width(dtype);
   if (dtype == FFSIM_C128)
     launch_fiber<double>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
-                         los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, ctx.lds_max, W,
+                         los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, ctx.lds_max, W,
                          stream);
   else
     launch_fiber<float>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
-                        los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, ctx.lds_max, W,
+                        los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, ctx.lds_max, W,
                         stream);
 }
 
@@ -538,6 +591,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
   const DevStrings& ds = dev_strings(ctx, norb, nocc);
   const DevGroups& gr = dev_groups(ctx, std::max(w, 1));
   const bool need_diag = !rot.trivial_diag;
+  require(!need_diag || norb <= kMaxDiagOrbitals, FFSIM_ERR_VALUE, "norb above 64");
   size_t diag_off = 0;
   if (need_diag) {
     std::vector<double> dd(2 * norb);
@@ -583,7 +637,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     launches.push_back([=, &ctx, &prog, &win, &gr, &ds]() {
       fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, gr, los, gd_cap,
                  gm_cap, prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff), n_layers,
-                 dg ? prog.dev<double>(diag_off) : nullptr, ds.d, ctx, stream);
+                 dg ? prog.dev<double>(diag_off) : nullptr, ds.d, norb, ctx, stream);
     });
   }
 }
@@ -949,6 +1003,66 @@ int ffsim_contract_num_op_sum(const void* vec, void* out, int dtype, int norb, i
     s.lin_col = vec_or_empty(coeffs_b, norb);
     diag_impl(ctx, accumulate ? DIAG_CONTRACT_ACC : DIAG_CONTRACT, vec, out, dtype, norb,
               n_alpha, n_beta, s, (hipStream_t)stream);
+  });
+}
+
+int ffsim_contract_one_body(const void* vec, void* out, int dtype, int norb, int n_alpha,
+                             int n_beta, const double* mat, double scale, int accumulate,
+                             void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(vec != out, FFSIM_ERR_VALUE, "out must not alias vec");
+    require(norb <= 64, FFSIM_ERR_VALUE, "norb above 64");
+    Context& ctx = context();
+    const long da = (long)binom(norb, n_alpha), db = (long)binom(norb, n_beta);
+    hipStream_t st = (hipStream_t)stream;
+    if (da == 0 || db == 0) return;
+    require(da < (1l << 31) && db < (1l << 31), FFSIM_ERR_VALUE, "dimension too large");
+    Program prog(ctx, st);
+    const size_t moff = prog.append(mat, sizeof(double) * 2 * norb * norb);
+    prog.commit();
+    const double* dm = prog.dev<double>(moff);
+    bool first = true;
+    for (int sector = 0; sector < 2; ++sector) {
+      const int nocc = sector == 0 ? n_alpha : n_beta;
+      const DevLinks& L = dev_links(ctx, norb, nocc);
+      const int acc = first ? accumulate : 1;
+      if (L.n_links == 0) {
+        if (first && !accumulate) {
+          // no electrons in this sector: its term is zero
+          const long n = da * db * (dtype == FFSIM_C128 ? 2 : 1);
+          hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<long>((n + 255) / 256, 65536)),
+                             dim3(256), 0, st, (double*)out, n);
+          first = false;
+        }
+        continue;
+      }
+      const dim3 grid = sector == 0 ? dim3((unsigned)da, (unsigned)((db + 255) / 256))
+                                    : dim3((unsigned)((db + 255) / 256), (unsigned)da);
+      if (dtype == FFSIM_C128) {
+        if (sector == 0)
+          hipLaunchKernelGGL((one_body_kernel<double, 0>), grid, dim3(256), 0, st,
+                             (const Cx<double>*)vec, (Cx<double>*)out, (int)da, (int)db, L.d,
+                             L.n_links, dm, norb, scale, acc);
+        else
+          hipLaunchKernelGGL((one_body_kernel<double, 1>), grid, dim3(256), 0, st,
+                             (const Cx<double>*)vec, (Cx<double>*)out, (int)da, (int)db, L.d,
+                             L.n_links, dm, norb, scale, acc);
+      } else {
+        if (sector == 0)
+          hipLaunchKernelGGL((one_body_kernel<float, 0>), grid, dim3(256), 0, st,
+                             (const Cx<float>*)vec, (Cx<float>*)out, (int)da, (int)db, L.d,
+                             L.n_links, dm, norb, scale, acc);
+        else
+          hipLaunchKernelGGL((one_body_kernel<float, 1>), grid, dim3(256), 0, st,
+                             (const Cx<float>*)vec, (Cx<float>*)out, (int)da, (int)db, L.d,
+                             L.n_links, dm, norb, scale, acc);
+      }
+      HIP_CHECK(hipGetLastError());
+      first = false;
+    }
+    prog.finish();
   });
 }
 

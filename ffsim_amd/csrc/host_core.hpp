@@ -133,6 +133,8 @@ struct PlannedPass {
 // brickwork layer pairs: an even layer (pairs p = 0, 2, 4 of the sub-window)
 // followed by an odd layer (p = 1, 3); slot p of a layer pair holds the
 // rotation on (p, p + 1) or is empty (bit p of `mask` clear).
+// Orbitals whose phases a fiber pass can apply (strings are 64-bit masks).
+constexpr int kMaxDiagOrbitals = 64;
 constexpr int kMaxSubWindow = 6;
 constexpr int kSlots = kMaxSubWindow - 1;
 

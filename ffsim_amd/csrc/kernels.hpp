@@ -75,13 +75,17 @@ constexpr int binom_const(int n, int k) {
   return r;
 }
 
+// x' = c x + s y, y' = c y - conj(s) x.  The terms of y' that need the old x are
+// formed first, so that both outputs can overwrite their inputs in place (no
+// register copies when the update sits on a conditional path).
 template <typename T>
 __device__ __forceinline__ void rot_pair(Cx<T>& a, Cx<T>& b, T c, T sr, T si) {
-  const Cx<T> x = a, y = b;
-  a.re = c * x.re + (sr * y.re - si * y.im);
-  a.im = c * x.im + (sr * y.im + si * y.re);
-  b.re = c * y.re - (sr * x.re + si * x.im);
-  b.im = c * y.im - (sr * x.im - si * x.re);
+  const T ur = sr * a.re + si * a.im;
+  const T ui = sr * a.im - si * a.re;
+  a.re = c * a.re + (sr * b.re - si * b.im);
+  a.im = c * a.im + (sr * b.im + si * b.re);
+  b.re = c * b.re - ur;
+  b.im = c * b.im - ui;
 }
 
 template <typename T, int W2, int K2, int P>
@@ -94,9 +98,10 @@ __device__ __forceinline__ void rot_sub(Cx<T>* v, T c, T sr, T si) {
 }
 
 template <typename T, int W2, int K2, int P>
-__device__ __forceinline__ void slot_apply(Cx<T>* v, const LayerPair& L, int mask) {
+__device__ __forceinline__ void slot_apply(Cx<T>* v, const double* c, const double* sr,
+                                           const double* si, int mask) {
   if constexpr (P + 1 < W2) {
-    if (mask & (1 << P)) rot_sub<T, W2, K2, P>(v, (T)L.c[P], (T)L.sr[P], (T)L.si[P]);
+    if (mask & (1 << P)) rot_sub<T, W2, K2, P>(v, (T)c[P], (T)sr[P], (T)si[P]);
   }
 }
 
@@ -113,13 +118,21 @@ __device__ __forceinline__ void group_apply(Cx<T>* tile, int S, int j,
 #pragma unroll
   for (int m = 0; m < N; ++m) v[m] = tile[idx[m]];
   for (int l = 0; l < n_lp; ++l) {
+    // one batch of scalar loads for the whole layer pair (uniform over the wave)
     const LayerPair& L = lp[l];
     const int mask = __builtin_amdgcn_readfirstlane(L.mask);
-    slot_apply<T, W2, K2, 0>(v, L, mask);
-    slot_apply<T, W2, K2, 2>(v, L, mask);
-    slot_apply<T, W2, K2, 4>(v, L, mask);
-    slot_apply<T, W2, K2, 1>(v, L, mask);
-    slot_apply<T, W2, K2, 3>(v, L, mask);
+    double c[kSlots], sr[kSlots], si[kSlots];
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+      c[q] = L.c[q];
+      sr[q] = L.sr[q];
+      si[q] = L.si[q];
+    }
+    slot_apply<T, W2, K2, 0>(v, c, sr, si, mask);
+    slot_apply<T, W2, K2, 2>(v, c, sr, si, mask);
+    slot_apply<T, W2, K2, 4>(v, c, sr, si, mask);
+    slot_apply<T, W2, K2, 1>(v, c, sr, si, mask);
+    slot_apply<T, W2, K2, 3>(v, c, sr, si, mask);
   }
 #pragma unroll
   for (int m = 0; m < N; ++m) tile[idx[m]] = v[m];
@@ -188,7 +201,7 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size
 // share their fiber and its staged tables.  While a tile is transformed in
 // LDS, the next tile's loads are already in flight into registers.
 //
-// LDS: tile[max_rows * S] | rows[max_rows] | layers | subs | slots | desc | members
+// LDS: tile[max_rows * S] | rows[max_rows] | slots | desc | members | wave schedule
 template <typename T, int W, int MODE, int NTHR, int W2>
 __global__ void __launch_bounds__(NTHR)
     fiber_kernel(const Cx<T>* src, Cx<T>* dst, long ld_src, long ld_dst, int n_batch, int tstore,
@@ -197,7 +210,7 @@ __global__ void __launch_bounds__(NTHR)
                  const SubPass* __restrict__ subs_g, int n_sub,
                  const LayerPair* __restrict__ layers_g, int n_layers,
                  const double* __restrict__ diag, const uint64_t* __restrict__ strings,
-                 int max_rows, int gd_cap, int gm_cap) {
+                 int n_diag, int max_rows, int gd_cap, int gm_cap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int S = (W >= 8) ? W + 1 : W;  // LDS stride of one string's batch
   constexpr int RSTEP = NTHR / W;          // rows per sweep in MODE_ROWS
@@ -207,10 +220,6 @@ __global__ void __launch_bounds__(NTHR)
   off += align16((size_t)max_rows * S * sizeof(Cx<T>));
   int32_t* rows = reinterpret_cast<int32_t*>(smem_raw + off);
   off += align16((size_t)max_rows * sizeof(int32_t));
-  LayerPair* layers = reinterpret_cast<LayerPair*>(smem_raw + off);
-  off += (size_t)n_layers * sizeof(LayerPair);
-  SubPass* subs = reinterpret_cast<SubPass*>(smem_raw + off);
-  off += (size_t)n_sub * sizeof(SubPass);
   GroupSlot* gslot = reinterpret_cast<GroupSlot*>(smem_raw + off);
   off += (size_t)kMaxLoSlots * sizeof(GroupSlot);
   uint32_t* gdesc = reinterpret_cast<uint32_t*>(smem_raw + off);
@@ -222,17 +231,34 @@ __global__ void __launch_bounds__(NTHR)
   int32_t* wsched = reinterpret_cast<int32_t*>(smem_raw + off);
   off += (size_t)kMaxLoSlots * NW * 2 * sizeof(int32_t);
   uint32_t* wchunks = reinterpret_cast<uint32_t*>(smem_raw + off);
+  off += align16((size_t)gd_cap * sizeof(uint32_t));
+  // Passes that apply orbital phases (diag != nullptr) also hold the phases and
+  // the fiber's row strings (32-bit when norb <= 32) in LDS, so that the store
+  // phase issues no global loads: a global load there would wait for every
+  // store issued before it.  Only those launches allocate this region.
+  double* dph = reinterpret_cast<double*>(smem_raw + off);
+  off += (size_t)2 * kMaxDiagOrbitals * sizeof(double);
+  uint32_t* rstr32 = reinterpret_cast<uint32_t*>(smem_raw + off);
+  uint64_t* rstr64 = reinterpret_cast<uint64_t*>(smem_raw + off);
+  const bool narrow_str = n_diag <= 32;
 
   const int tid = threadIdx.x;
   const int n_bt = (n_batch + W - 1) / W;
   const long n_tiles = (long)n_bt * n_fibers;
 
-  for (int k = tid; k < n_layers; k += NTHR) layers[k] = layers_g[k];
-  for (int k = tid; k < n_sub; k += NTHR) subs[k] = subs_g[k];
 
   // stage the row addresses and group tables of fiber f
   auto stage_fiber = [&](const Fiber& fd) {
     for (int i = tid; i < fd.n_rows; i += NTHR) rows[i] = row_table[fd.row_off + i];
+    if (diag) {
+      for (int i = tid; i < fd.n_rows; i += NTHR) {
+        const uint64_t sv = strings[row_table[fd.row_off + i]];
+        if (narrow_str)
+          rstr32[i] = (uint32_t)sv;
+        else
+          rstr64[i] = sv;
+      }
+    }
     if (n_sub > 0) {
       int d_base = 0, m_base = 0;
       for (int u = 0; u < los.n; ++u) {
@@ -298,27 +324,28 @@ __global__ void __launch_bounds__(NTHR)
       }
     }
   };
-  // rows not covered by one register chunk (only for fibers above CH * RSTEP rows)
-  auto load_rest = [&](int nr, int b0, int nb) {
-    const int covered = (MODE == MODE_ROWS) ? CH * RSTEP : (CH / W) * NTHR;
-    if (nr <= covered) return;
+  // rows [first, nr) straight from global memory into LDS
+  constexpr int kCovered = (MODE == MODE_ROWS) ? CH * RSTEP : (CH / W) * NTHR;
+  auto load_rows = [&](int first, int nr, int b0, int nb) {
+    if (nr <= first) return;
     if (MODE == MODE_ROWS) {
       const int j = tid % W, i0 = tid / W;
-      for (int i = covered + i0; i < nr; i += RSTEP)
+#pragma unroll 4
+      for (int i = first + i0; i < nr; i += RSTEP)
         tile[i * S + j] = (j < nb) ? src[(long)rows[i] * ld_src + b0 + j] : Cx<T>{T(0), T(0)};
     } else {
-      for (int i = covered + tid; i < nr; i += NTHR)
+      for (int i = first + tid; i < nr; i += NTHR)
         for (int j = 0; j < W; ++j)
           tile[i * S + j] = (j < nb) ? src[(long)(b0 + j) * ld_src + rows[i]] : Cx<T>{T(0), T(0)};
     }
   };
   auto row_phase = [&](int i, T& pr, T& pim) {
     double re = 1.0, im = 0.0;
-    uint64_t bits = strings[rows[i]];
+    uint64_t bits = narrow_str ? (uint64_t)rstr32[i] : rstr64[i];
     while (bits) {
       const int o = __builtin_ctzll(bits);
       bits &= bits - 1;
-      const double dr = diag[2 * o], di = diag[2 * o + 1];
+      const double dr = dph[2 * o], di = dph[2 * o + 1];
       const double t = re * dr - im * di;
       im = re * di + im * dr;
       re = t;
@@ -364,8 +391,18 @@ __global__ void __launch_bounds__(NTHR)
     }
   };
 
-  long t = blockIdx.x;
+  // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so block
+  // b runs on XCD b % 8.  Renumbering makes each XCD's blocks take consecutive
+  // tiles, i.e. neighbouring batch tiles of one fiber, whose rows share the
+  // 128-byte lines that straddle tile borders; those lines then hit in that
+  // XCD's L2 instead of being fetched twice from HBM.
+  const int G = gridDim.x;
+  const int bid = (G & 7) == 0 ? (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3)
+                               : (int)blockIdx.x;
+  long t = bid;
   if (t >= n_tiles) return;
+  if (diag)
+    for (int k = tid; k < 2 * n_diag; k += NTHR) dph[k] = diag[k];
   int f = (int)(t / n_bt);
   Fiber fd = fibers[f];
   stage_fiber(fd);
@@ -374,9 +411,7 @@ __global__ void __launch_bounds__(NTHR)
   {
     const int b0 = (int)(t % n_bt) * W;
     const int nb = min(W, n_batch - b0);
-    load_regs(buf, fd.n_rows, b0, nb);
-    regs_to_lds(buf, fd.n_rows);
-    load_rest(fd.n_rows, b0, nb);
+    load_rows(0, fd.n_rows, b0, nb);
   }
   while (true) {
     const int nr = fd.n_rows;
@@ -387,16 +422,20 @@ __global__ void __launch_bounds__(NTHR)
     const bool prefetch = (fn == f);
     const int b0n = (int)(tn % n_bt) * W;
     const int nbn = prefetch ? min(W, n_batch - b0n) : 0;
-    if (prefetch) load_regs(buf, nr, b0n, nbn);
+    // unconditional (no loads when not prefetching) so that buf is never carried
+    // around the loop: a loop-carried buf makes the compiler copy it at the back
+    // edge behind a vmcnt(0) that would also drain this tile's stores
+    load_regs(buf, prefetch ? nr : 0, b0n, nbn);
 
     // ---- register sub-passes on the current tile
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
     const int lane = tid % 64;
     for (int sp = 0; sp < n_sub; ++sp) {
       __syncthreads();
-      const SubPass P = subs[sp];
+      // the pass program is uniform: scalar loads straight from device memory
+      const SubPass P = subs_g[sp];
       const GroupSlot gs = gslot[P.slot];
-      const LayerPair* lp = layers + P.lp0;
+      const LayerPair* lp = layers_g + P.lp0;
       const int c0 = wsched[(P.slot * NW + wave) * 2];
       const int nc = wsched[(P.slot * NW + wave) * 2 + 1];
       for (int c = 0; c < nc; ++c) {
@@ -408,21 +447,23 @@ __global__ void __launch_bounds__(NTHR)
         }
       }
     }
+    // Only the prefetch loads are outstanding here.  Waiting for them now lets the
+    // stores below stay in flight while the prefetched tile is written to LDS
+    // (vmcnt counts stores too: a wait placed after them would drain them).
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt/lgkmcnt unchanged
     __syncthreads();
     store_tile(nr, b0, nb);
     __syncthreads();
     if (tn >= n_tiles) break;
     if (prefetch) {
       regs_to_lds(buf, nr);
-      load_rest(nr, b0n, nbn);
+      load_rows(kCovered, nr, b0n, nbn);
     } else {
       f = fn;
       fd = fibers[f];
       stage_fiber(fd);
       __syncthreads();
-      load_regs(buf, fd.n_rows, b0n, min(W, n_batch - b0n));
-      regs_to_lds(buf, fd.n_rows);
-      load_rest(fd.n_rows, b0n, min(W, n_batch - b0n));
+      load_rows(0, fd.n_rows, b0n, min(W, n_batch - b0n));
     }
     t = tn;
   }
@@ -692,6 +733,50 @@ __global__ void __launch_bounds__(256)
       __syncthreads();
     }
   }
+}
+
+// One-body contraction out (+)= scale * sum_pq M_pq a+_p a_q psi, one spin sector
+// per launch, as a gather over link tables: string T's links are the L strings
+// S with a+_p a_q |S> = sign |T> (p in T, q = p or q not in T), each stored as
+// (S, p | q << 8 | (sign < 0) << 16).  m: norb x norb complex (re, im pairs).
+// SECTOR 0: strings index rows, one block row T x 256 columns, the links of T
+//   are uniform over the block (scalar loads), reads of psi rows coalesced.
+// SECTOR 1: strings index columns, thread = (row a, column T).
+// accumulate: 0 -> out = scale acc, 1 -> out += scale acc.
+template <typename T, int SECTOR>
+__global__ void __launch_bounds__(256)
+    one_body_kernel(const Cx<T>* __restrict__ psi, Cx<T>* __restrict__ out, int n_rows,
+                    int n_cols, const int32_t* __restrict__ links, int n_links,
+                    const double* __restrict__ m, int norb, double scale, int accumulate) {
+  int row, col;
+  if (SECTOR == 0) {
+    row = blockIdx.x;
+    col = blockIdx.y * 256 + threadIdx.x;
+  } else {
+    row = blockIdx.y;
+    col = blockIdx.x * 256 + threadIdx.x;
+  }
+  if (row >= n_rows || col >= n_cols) return;
+  const int tstr = SECTOR == 0 ? row : col;
+  const int32_t* lk = links + (long)tstr * n_links * 2;
+  double ar = 0.0, ai = 0.0;
+  for (int l = 0; l < n_links; ++l) {
+    const int src = lk[2 * l];
+    const int info = lk[2 * l + 1];
+    const int p = info & 0xff, q = (info >> 8) & 0xff;
+    const double sg = (info >> 16) & 1 ? -1.0 : 1.0;
+    const double mr = sg * m[2 * (p * norb + q)], mi = sg * m[2 * (p * norb + q) + 1];
+    const Cx<T> x = SECTOR == 0 ? psi[(long)src * n_cols + col] : psi[(long)row * n_cols + src];
+    ar += mr * (double)x.re - mi * (double)x.im;
+    ai += mr * (double)x.im + mi * (double)x.re;
+  }
+  const long o = (long)row * n_cols + col;
+  double r = scale * ar, i = scale * ai;
+  if (accumulate) {
+    r += (double)out[o].re;
+    i += (double)out[o].im;
+  }
+  out[o] = Cx<T>{(T)r, (T)i};
 }
 
 __global__ void fill_zero_kernel(double* __restrict__ x, long n) {

@@ -300,7 +300,7 @@ def ucj_state(params, reference_state, *, norb: int, nelec, n_reps: int,
 
 
 class HermitianApply(torch.autograd.Function):
-    """y = H x for a Hermitian device linear operator (backward: H g)."""
+    """y = H x for a device linear operator (backward: H^dag g, i.e. H g when Hermitian)."""
 
     @staticmethod
     def forward(ctx, x, linop):
@@ -309,7 +309,8 @@ class HermitianApply(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        return ctx.linop.matvec_device(g.reshape(-1).contiguous().clone()), None
+        adjoint = getattr(ctx.linop, "rmatvec_device", None) or ctx.linop.matvec_device
+        return adjoint(g.reshape(-1).contiguous().clone()), None
 
 
 def expectation(psi: torch.Tensor, linop) -> torch.Tensor:

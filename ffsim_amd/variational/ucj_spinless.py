@@ -1,0 +1,174 @@
+"""Spinless (L)UCJ operator (python/ffsim/variational/ucj_spinless.py).
+
+One orbital rotation and one diagonal Coulomb matrix per repetition.  On a
+spinless state (``nelec`` an int) the gates act on the single sector; on a
+spinful state the same rotation acts on both spins and the diagonal Coulomb
+operator is (J, 0, J), as in ucj_spinless.py:456-515.
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import InitVar, dataclass
+
+import numpy as np
+
+from ffsim_amd import _device
+from ffsim_amd import linalg as _linalg
+from ffsim_amd.gates.diag_coulomb import diag_coulomb_evolution_in_place
+from ffsim_amd.gates.orbital_rotation import orbital_rotation_in_place
+from ffsim_amd.variational.ucj_spin_balanced import _validate_interaction_pairs
+
+
+@dataclass(frozen=True)
+class UCJOpSpinless:
+    r"""A spinless unitary cluster Jastrow operator."""
+
+    diag_coulomb_mats: np.ndarray  # (n_reps, norb, norb)
+    orbital_rotations: np.ndarray  # (n_reps, norb, norb)
+    final_orbital_rotation: np.ndarray | None = None  # (norb, norb)
+    validate: InitVar[bool] = True
+    rtol: InitVar[float] = 1e-5
+    atol: InitVar[float] = 1e-8
+
+    def __post_init__(self, validate: bool, rtol: float, atol: float):
+        """Validation of ucj_spinless.py:76-118."""
+        if not validate:
+            return
+        if self.diag_coulomb_mats.ndim != 3:
+            raise ValueError(
+                "diag_coulomb_mats should have shape (n_reps, norb, norb). "
+                f"Got shape {self.diag_coulomb_mats.shape}."
+            )
+        if self.orbital_rotations.ndim != 3:
+            raise ValueError(
+                "orbital_rotations should have shape (n_reps, norb, norb). "
+                f"Got shape {self.orbital_rotations.shape}."
+            )
+        if self.final_orbital_rotation is not None and self.final_orbital_rotation.ndim != 2:
+            raise ValueError(
+                "final_orbital_rotation should have shape (norb, norb). "
+                f"Got shape {self.final_orbital_rotation.shape}."
+            )
+        if self.diag_coulomb_mats.shape[0] != self.orbital_rotations.shape[0]:
+            raise ValueError(
+                "diag_coulomb_mats and orbital_rotations should have the same first "
+                f"dimension. Got {self.diag_coulomb_mats.shape[0]} and "
+                f"{self.orbital_rotations.shape[0]}."
+            )
+        if not all(_linalg.is_real_symmetric(m, rtol=rtol, atol=atol)
+                   for m in self.diag_coulomb_mats):
+            raise ValueError("Diagonal Coulomb matrices were not all real symmetric.")
+        if not all(_linalg.is_unitary(u, rtol=rtol, atol=atol) for u in self.orbital_rotations):
+            raise ValueError("Orbital rotations were not all unitary.")
+        if self.final_orbital_rotation is not None and not _linalg.is_unitary(
+            self.final_orbital_rotation, rtol=rtol, atol=atol
+        ):
+            raise ValueError("Final orbital rotation was not unitary.")
+
+    @property
+    def norb(self) -> int:
+        return self.diag_coulomb_mats.shape[-1]
+
+    @property
+    def n_reps(self) -> int:
+        return self.diag_coulomb_mats.shape[0]
+
+    @staticmethod
+    def n_params(norb: int, n_reps: int, *, interaction_pairs=None,
+                 with_final_orbital_rotation: bool = False) -> int:
+        """ucj_spinless.py:131-173."""
+        _validate_interaction_pairs(interaction_pairs, ordered=False)
+        n_dc = norb * (norb + 1) // 2 if interaction_pairs is None else len(interaction_pairs)
+        return n_reps * (n_dc + norb**2) + with_final_orbital_rotation * norb**2
+
+    @staticmethod
+    def from_parameters(params: np.ndarray, *, norb: int, n_reps: int, interaction_pairs=None,
+                        with_final_orbital_rotation: bool = False) -> "UCJOpSpinless":
+        """ucj_spinless.py:175-255."""
+        n_params = UCJOpSpinless.n_params(
+            norb, n_reps, interaction_pairs=interaction_pairs,
+            with_final_orbital_rotation=with_final_orbital_rotation,
+        )
+        if len(params) != n_params:
+            raise ValueError(
+                "The number of parameters passed did not match the number expected "
+                "based on the function inputs. "
+                f"Expected {n_params} but got {len(params)}."
+            )
+        pairs = (list(itertools.combinations_with_replacement(range(norb), 2))
+                 if interaction_pairs is None else list(interaction_pairs))
+        mats = np.zeros((n_reps, norb, norb))
+        rots = np.zeros((n_reps, norb, norb), dtype=complex)
+        index = 0
+        for rep in range(n_reps):
+            rots[rep] = _linalg.unitary_from_parameters(params[index : index + norb**2], norb)
+            index += norb**2
+            if pairs:
+                rows, cols = zip(*pairs)
+                vals = params[index : index + len(pairs)]
+                mats[rep][cols, rows] = vals
+                mats[rep][rows, cols] = vals
+                index += len(pairs)
+        final = None
+        if with_final_orbital_rotation:
+            final = _linalg.unitary_from_parameters(params[index:], norb)
+        return UCJOpSpinless(diag_coulomb_mats=mats, orbital_rotations=rots,
+                             final_orbital_rotation=final)
+
+    def to_parameters(self, *, interaction_pairs=None) -> np.ndarray:
+        """ucj_spinless.py:257-320."""
+        n_reps, norb, _ = self.diag_coulomb_mats.shape
+        n_params = UCJOpSpinless.n_params(
+            norb, n_reps, interaction_pairs=interaction_pairs,
+            with_final_orbital_rotation=self.final_orbital_rotation is not None,
+        )
+        pairs = (list(itertools.combinations_with_replacement(range(norb), 2))
+                 if interaction_pairs is None else list(interaction_pairs))
+        params = np.zeros(n_params)
+        index = 0
+        for u, m in zip(self.orbital_rotations, self.diag_coulomb_mats):
+            params[index : index + norb**2] = _linalg.unitary_to_parameters(u)
+            index += norb**2
+            if pairs:
+                params[index : index + len(pairs)] = m[tuple(zip(*pairs))]
+                index += len(pairs)
+        if self.final_orbital_rotation is not None:
+            params[index:] = _linalg.unitary_to_parameters(self.final_orbital_rotation)
+        return params
+
+    def apply_in_place(self, t, norb: int, nelec):
+        """Apply to a resident 1-D device state in place (nelec int or pair)."""
+        spinless = isinstance(nelec, int)
+        sectors = (nelec, 0) if spinless else tuple(nelec)
+        zero = np.zeros((norb, norb))
+        current = np.eye(norb, dtype=complex)
+        for mat, rot in zip(self.diag_coulomb_mats, self.orbital_rotations):
+            step = rot.conj().T @ current
+            orbital_rotation_in_place(t, step, norb, nelec)
+            dc = (mat, None, None) if spinless else (mat, zero, mat)
+            diag_coulomb_evolution_in_place(t, dc, -1.0, norb, sectors, False)
+            current = rot
+        if self.final_orbital_rotation is not None:
+            current = self.final_orbital_rotation @ current
+        orbital_rotation_in_place(t, current, norb, nelec)
+        return t
+
+    def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
+        st = _device.to_device(vec, copy)
+        self.apply_in_place(st.tensor, norb, nelec)
+        return st.restore(st.tensor)
+
+    def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
+        if isinstance(other, UCJOpSpinless):
+            if not np.allclose(self.diag_coulomb_mats, other.diag_coulomb_mats, rtol=rtol, atol=atol):
+                return False
+            if not np.allclose(self.orbital_rotations, other.orbital_rotations, rtol=rtol, atol=atol):
+                return False
+            if (self.final_orbital_rotation is None) != (other.final_orbital_rotation is None):
+                return False
+            if self.final_orbital_rotation is not None:
+                return np.allclose(self.final_orbital_rotation, other.final_orbital_rotation,
+                                   rtol=rtol, atol=atol)
+            return True
+        return NotImplemented

@@ -138,6 +138,14 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
                                   const double* orbital_rotations,
                                   const double* final_orbital_rotation, void* stream);
 
+/* One-body contraction: out (+)= scale * sum_{pq, sigma} mat[p][q] a+_{p sigma} a_{q sigma} vec,
+ * mat a HOST norb x norb complex128 matrix.  Replaces contract_one_body
+ * (python/ffsim/contract/one_body.py:24-56, i.e. pyscf's contract_1e over the link
+ * tables of _cistring.py:33-42).  out must not alias vec.  norb <= 64. */
+int ffsim_contract_one_body(const void* vec, void* out, int dtype, int norb, int n_alpha,
+                            int n_beta, const double* mat, double scale, int accumulate,
+                            void* stream);
+
 /* ---- adjoint reductions (differentiation; no counterpart in the reference,
  * whose only gradient is finite differences, python/ffsim/optimize/_util.py:60-110) */
 

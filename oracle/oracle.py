@@ -601,6 +601,37 @@ def apply_ucj_spin_balanced(vec, diag_coulomb_mats, orbital_rotations, final_orb
     return apply_orbital_rotation(vec, last, norb, nelec, copy=False)
 
 
+def apply_ucj_spin_unbalanced(vec, diag_coulomb_mats, orbital_rotations, final_orbital_rotation,
+                              norb, nelec):
+    """UCJOpSpinUnbalanced._apply_unitary_ (python/ffsim/variational/ucj_spin_unbalanced.py:705-745)."""
+    vec = np.array(vec, dtype=np.complex128)
+    eye = np.eye(norb)
+    current = np.stack([eye, eye]).astype(complex)
+    for mats, rots in zip(diag_coulomb_mats, orbital_rotations):
+        step = rots.transpose(0, 2, 1).conj() @ current
+        vec = apply_orbital_rotation(vec, (step[0], step[1]), norb, nelec, copy=False)
+        vec = apply_diag_coulomb_evolution(vec, (mats[0], mats[1], mats[2]), -1.0, norb, nelec,
+                                           copy=False)
+        current = rots
+    last = current if final_orbital_rotation is None else final_orbital_rotation @ current
+    return apply_orbital_rotation(vec, (last[0], last[1]), norb, nelec, copy=False)
+
+
+def apply_ucj_spinless(vec, diag_coulomb_mats, orbital_rotations, final_orbital_rotation, norb,
+                       nelec):
+    """UCJOpSpinless._apply_unitary_ (python/ffsim/variational/ucj_spinless.py:456-515)."""
+    vec = np.array(vec, dtype=np.complex128)
+    current = np.eye(norb, dtype=complex)
+    zero = np.zeros((norb, norb))
+    for mat, rot in zip(diag_coulomb_mats, orbital_rotations):
+        vec = apply_orbital_rotation(vec, rot.T.conj() @ current, norb, nelec, copy=False)
+        dc = mat if isinstance(nelec, int) else (mat, zero, mat)
+        vec = apply_diag_coulomb_evolution(vec, dc, -1.0, norb, nelec, copy=False)
+        current = rot
+    last = current if final_orbital_rotation is None else final_orbital_rotation @ current
+    return apply_orbital_rotation(vec, last, norb, nelec, copy=False)
+
+
 def ucj_energy(params, hamiltonian_matvec, reference_state, norb, nelec, n_reps,
                interaction_pairs=None, with_final_orbital_rotation=False) -> float:
     """<psi(theta)|H|psi(theta)> for the spin-balanced UCJ ansatz on the CPU path."""
@@ -657,6 +688,58 @@ def transition_rdm(lam, psi, norb, nelec) -> np.ndarray:
                         val = np.vdot(lam[:, t], psi[:, k])
                     out[sigma, p, q] += sign * val
     return out
+
+
+def cre_des_sign(p: int, q: int, string: int) -> int:
+    """Sign of a+_p a_q on a string: parity of the occupied orbitals strictly
+    between p and q (pyscf.fci.cistring.cre_des_sign, pyscf 2.14.0)."""
+    if p == q:
+        return 1
+    lo, hi = min(p, q), max(p, q)
+    mask = ((1 << hi) - 1) ^ ((1 << (lo + 1)) - 1)
+    return -1 if bin(string & mask).count("1") % 2 else 1
+
+
+def gen_linkstr_index(norb: int, nocc: int) -> np.ndarray:
+    """pyscf.fci.cistring.gen_linkstr_index (pyscf 2.14.0, cached by
+    python/ffsim/_cistring.py:33-36): per string, rows (a, i, address, sign),
+    first the occupied (i, i) entries, then every occupied i to virtual a."""
+    strings = cached_strings(norb, nocc)
+    index = {int(s): k for k, s in enumerate(strings)}
+    n_vir = norb - nocc
+    out = np.zeros((len(strings), nocc + nocc * n_vir, 4), dtype=np.int64)
+    for k, s in enumerate(strings):
+        s = int(s)
+        occ = [o for o in range(norb) if (s >> o) & 1]
+        vir = [o for o in range(norb) if not (s >> o) & 1]
+        row = 0
+        for i in occ:
+            out[k, row] = (i, i, k, 1)
+            row += 1
+        for i in occ:
+            for a in vir:
+                t = s ^ (1 << i) ^ (1 << a)
+                out[k, row] = (a, i, index[t], cre_des_sign(a, i, s))
+                row += 1
+    return out
+
+
+def contract_one_body(vec, mat, norb, nelec):
+    """python/ffsim/contract/one_body.py:24-56: sum_pq M_pq a+_p a_q on both spins,
+    as pyscf's contract_1e does it: for every source string and link (a, i, t, sign),
+    out[t] += sign * M[a, i] * vec[source]."""
+    na, nb = nelec
+    da, db = math.comb(norb, na), math.comb(norb, nb)
+    v = np.asarray(vec, dtype=np.complex128).reshape(da, db)
+    mat = np.asarray(mat, dtype=np.complex128)
+    out = np.zeros_like(v)
+    for src, links in enumerate(gen_linkstr_index(norb, na)):
+        for a, i, t, sign in links:
+            out[t] += sign * mat[a, i] * v[src]
+    for src, links in enumerate(gen_linkstr_index(norb, nb)):
+        for a, i, t, sign in links:
+            out[:, t] += sign * mat[a, i] * v[:, src]
+    return out.reshape(-1)
 
 
 def diag_coulomb_phase_reductions(psi, lam, norb, nelec):

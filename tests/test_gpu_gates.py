@@ -315,3 +315,21 @@ def test_trotter_double_factorized(order, n_steps, z_rep, rng):
     got = ffsim_amd.simulate_trotter_double_factorized(vec, df, 0.3, norb=norb, nelec=nelec,
                                                        n_steps=n_steps, order=order)
     np.testing.assert_allclose(got, expected, atol=1e-11, rtol=0)
+
+
+@pytest.mark.parametrize("norb,nelec", [(1, (1, 0)), (3, (2, 1)), (4, (0, 4)), (6, (3, 2)),
+                                        (8, (4, 4)), (10, (5, 3)), (12, (6, 6))])
+def test_contract_one_body(norb, nelec, rng):
+    import ffsim_amd
+
+    _case("one-body contraction", norb, nelec)
+    vec = _rand(norb, nelec, rng)
+    mat = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    expected = O.contract_one_body(vec, mat, norb, nelec)
+    got = ffsim_amd.contract.contract_one_body(vec, mat, norb, nelec)
+    np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)
+    op = ffsim_amd.contract.one_body_linop(mat, norb, nelec)
+    np.testing.assert_allclose(op @ vec, expected, atol=1e-12, rtol=0)
+    # rmatvec applies the conjugate transpose
+    np.testing.assert_allclose(op.rmatvec(vec), O.contract_one_body(vec, mat.T.conj(), norb, nelec),
+                               atol=1e-12, rtol=0)

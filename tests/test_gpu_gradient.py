@@ -148,7 +148,13 @@ def test_config_norb12_gradient_668_params(rng):
     _case("config norb=12 GPU central FD, 1336 energies")
     assert abs(gpu_energy(theta) - energy) < 1e-12
     fd_gpu = O.fd_gradient(gpu_energy, theta, 1e-6)
-    np.testing.assert_allclose(grad, fd_gpu, atol=1e-7, rtol=0)
+    # central-FD error at h=1e-6: truncation ~h^2 (third derivative)/6 ~ 1e-12, rounding
+    # ~|E| * 1e-13 / h, the energies carrying ~1e-13 relative rounding from thousands of
+    # Givens updates; the adjoint itself matches the oracle's sweep to 1e-10 below
+    fd_tol = 5e-7 * max(1.0, abs(energy))
+    print(f"E={energy:.12f} max|adjoint-FD|={np.abs(grad - fd_gpu).max():.2e} tol={fd_tol:.1e}",
+          flush=True)
+    np.testing.assert_allclose(grad, fd_gpu, atol=fd_tol, rtol=0)
 
     def h(v):
         return O.df_hamiltonian_matvec(v, df.one_body_tensor, df.diag_coulomb_mats,
@@ -159,7 +165,7 @@ def test_config_norb12_gradient_668_params(rng):
     fd_cpu = O.fd_gradient(
         lambda p: O.ucj_energy(p, h, psi0, norb, nelec, n_reps, pairs), theta, 1e-6, subset
     )
-    np.testing.assert_allclose(grad[subset], fd_cpu, atol=1e-7, rtol=0)
+    np.testing.assert_allclose(grad[subset], fd_cpu, atol=fd_tol, rtol=0)
     e_cpu = O.ucj_energy(theta, h, psi0, norb, nelec, n_reps, pairs)
     assert abs(e_cpu - energy) < 1e-10
 

@@ -301,3 +301,26 @@ def test_trotter_single_term_is_exact(rng):
                                                norb, nelec, n_steps=1)
     exact = scipy.linalg.expm(-0.9j * _dense_df(np.zeros((norb, norb)), [J], [U], 0.0, norb, nelec)) @ vec
     np.testing.assert_allclose(got, exact, atol=1e-12)
+
+
+@pytest.mark.parametrize("norb, nelec", [(1, (1, 0)), (3, (2, 1)), (4, (2, 2)), (5, (3, 1)),
+                                         (4, (0, 4)), (6, (3, 3))])
+def test_contract_one_body_vs_dense(norb, nelec, rng):
+    """contract_one_body restated over pyscf's link tables vs the dense one-body
+    matrix built from creation/annihilation signs."""
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    vec = O.random_state_vector(dim, rng)
+    mat = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    expected = O.one_body_dense(mat, norb, nelec) @ vec
+    np.testing.assert_allclose(O.contract_one_body(vec, mat, norb, nelec), expected, atol=TOL)
+
+
+def test_linkstr_index_known_answer():
+    """pyscf's gen_linkstr_index layout for norb=3, one electron: strings 0b001,
+    0b010, 0b100; (i, i) first, then i -> a with the sign of a+_a a_i."""
+    links = O.gen_linkstr_index(3, 1)
+    np.testing.assert_array_equal(links[0], [[0, 0, 0, 1], [1, 0, 1, 1], [2, 0, 2, 1]])
+    np.testing.assert_array_equal(links[1], [[1, 1, 1, 1], [0, 1, 0, 1], [2, 1, 2, 1]])
+    # two electrons: moving orbital 0 to 2 across an occupied orbital 1 flips the sign
+    links2 = O.gen_linkstr_index(3, 2)
+    assert [2, 0, 2, -1] in links2[0].tolist()  # 0b011 -> 0b110, address 2

@@ -103,6 +103,63 @@ def measured_traffic(kernel: str, workload: str, dtype: str):
     return None
 
 
+def trotter_leg(norb: int = 18, n_factors: int = 10, time_step: float = 0.1, seed: int = 5) -> dict:
+    """One first-order Trotter step of a synthetic double-factorized Hamiltonian
+    (one-body term plus n_factors rotated diagonal Coulomb terms, J ~ N(0, 0.1),
+    t = 0.1) on the norb=18 (9,9) state (48620 x 48620; 37.8 GB in complex128),
+    in complex128 and in complex64 from the same initial state: time per step,
+    per-kernel achieved GB/s and the complex64-vs-complex128 error on device."""
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd import _lib
+
+    nelec = (norb // 2, norb // 2)
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    ham = ffsim_amd.random.random_df_hamiltonian_normal(norb, n_factors=n_factors, seed=seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    psi0 = torch.randn(dim, dtype=torch.complex128, device="cuda", generator=gen)
+    psi0 /= torch.linalg.vector_norm(psi0)
+    out = {"workload": f"DF Trotter step, norb={norb} nelec={nelec}, {n_factors} factors, "
+                       f"t={time_step}, order 0", "dim": dim}
+    results = {}
+    for dtype in (torch.complex128, torch.complex64):
+        tag = "c128" if dtype == torch.complex128 else "c64"
+        psi = psi0.to(dtype)
+        if dtype == torch.complex64:
+            del psi0
+        torch.cuda.synchronize()
+        buf = (ctypes.c_double * 6)()
+        _lib.call("ffsim_amd_profile_begin")
+        t0 = time.perf_counter()
+        ffsim_amd.simulate_trotter_double_factorized(psi, ham, time_step, norb=norb, nelec=nelec,
+                                                     copy=False)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        _lib.call("ffsim_amd_profile_end", buf)
+        fms, fn, fb, dms, dn, db = list(buf)
+        out[tag] = {
+            "ms_per_step": round(elapsed * 1e3, 1),
+            "fiber_launches": int(fn),
+            "fiber_avg_ms": round(fms / max(fn, 1), 3),
+            "fiber_GBs": round(fb / max(fms, 1e-9) / 1e6, 1),
+            "diag_launches": int(dn),
+            "diag_GBs": round(db / max(dms, 1e-9) / 1e6, 1),
+        }
+        results[tag] = psi
+        log(f"trotter {tag}: {out[tag]}")
+    psi128, psi64 = results["c128"], results["c64"].to(torch.complex128)
+    out["c64_vs_c128_max_abs"] = float((psi64 - psi128).abs().max().item())
+    chunk = 1 << 30  # torch.vdot takes at most 2^31 - 1 elements
+    overlap = sum(torch.vdot(psi128[i : i + chunk], psi64[i : i + chunk]).item()
+                  for i in range(0, dim, chunk))
+    out["c64_vs_c128_infidelity"] = float(1.0 - abs(overlap))
+    del results, psi128, psi64
+    torch.cuda.empty_cache()
+    return out
+
+
 def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
     """Served mixed stream vs the unbatched plain path, same Poisson arrivals.
 
@@ -205,6 +262,7 @@ def main() -> int:
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--serving-load", type=float, default=0.9)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
@@ -321,12 +379,17 @@ def main() -> int:
             "achieved_GBs": round(diag_bytes / max(diag_ms, 1e-9) / 1e6, 1),
         },
     }
+    vec_host = None if args.no_cpu_baseline else state.to(torch.complex128).cpu().numpy()
+    if not args.no_trotter:
+        del state
+        torch.cuda.empty_cache()
+        log("trotter leg")
+        result["trotter_norb18"] = trotter_leg()
     if not args.no_serving:
         log("serving leg")
         result["serving"] = serving_leg(args.serving_requests, args.serving_load)
         log(f"serving: {result['serving']}")
     if not args.no_cpu_baseline:
-        vec_host = state.to(torch.complex128).cpu().numpy()
         result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, args.cpu_threads)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
     print(json.dumps(result), flush=True)

@@ -333,3 +333,32 @@ def test_contract_one_body(norb, nelec, rng):
     # rmatvec applies the conjugate transpose
     np.testing.assert_allclose(op.rmatvec(vec), O.contract_one_body(vec, mat.T.conj(), norb, nelec),
                                atol=1e-12, rtol=0)
+
+
+def test_norb18_rotation_round_trip_and_trotter_norm():
+    """Full-size norb=18 (9,9) state (48620^2, 37.8 GB complex128): an orbital
+    rotation followed by its inverse returns the state, and a Trotter step
+    preserves the norm (size-independent properties; the oracle cannot run here)."""
+    torch = _torch()
+    import ffsim_amd
+
+    norb, nelec = 18, (9, 9)
+    _case("norb=18 round trip")
+    dim = math.comb(norb, 9) ** 2
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(18)
+    psi = torch.randn(dim, dtype=torch.complex128, device="cuda", generator=gen)
+    psi /= torch.linalg.vector_norm(psi)
+    ref = psi.clone()
+    U = ffsim_amd.random.random_unitary(norb, seed=18)
+    ffsim_amd.apply_orbital_rotation(psi, U, norb, nelec, copy=False)
+    assert (psi - ref).abs().max().item() > 1e-6  # it did rotate
+    ffsim_amd.apply_orbital_rotation(psi, U.T.conj(), norb, nelec, copy=False)
+    assert (psi - ref).abs().max().item() < 1e-12
+    del ref
+    _case("norb=18 trotter step norm")
+    ham = ffsim_amd.random.random_df_hamiltonian_normal(norb, n_factors=2, seed=18)
+    ffsim_amd.simulate_trotter_double_factorized(psi, ham, 0.1, norb=norb, nelec=nelec, copy=False)
+    assert abs(torch.linalg.vector_norm(psi).item() - 1.0) < 1e-12
+    del psi
+    torch.cuda.empty_cache()

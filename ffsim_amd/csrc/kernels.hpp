@@ -88,6 +88,47 @@ __device__ __forceinline__ void rot_pair(Cx<T>& a, Cx<T>& b, T c, T sr, T si) {
   b.im = c * b.im - ui;
 }
 
+// FP64: the same twelve operations as AMD instructions whose outputs are tied to
+// their inputs, so both elements are updated in their own registers.  Left to
+// itself the compiler renames them and, on the conditional slot paths, copies
+// them back with v_mov_b64 (2 copies per pair, ~17% of the VALU stream).
+template <>
+__device__ __forceinline__ void rot_pair<double>(Cx<double>& a, Cx<double>& b, double c, double sr,
+                                                 double si) {
+  double ur, ui, t1, t2;
+  asm("v_mul_f64 %0, %1, %2" : "=v"(ur) : "s"(sr), "v"(a.re));
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(ur) : "s"(si), "v"(a.im));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(ui) : "s"(sr), "v"(a.im));
+  asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(ui) : "s"(si), "v"(a.re));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(t1) : "s"(si), "v"(b.im));
+  asm("v_fma_f64 %0, %1, %2, -%0" : "+v"(t1) : "s"(sr), "v"(b.re));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(t2) : "s"(si), "v"(b.re));
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(t2) : "s"(sr), "v"(b.im));
+  asm("v_fma_f64 %0, %1, %0, %2" : "+v"(a.re) : "s"(c), "v"(t1));
+  asm("v_fma_f64 %0, %1, %0, %2" : "+v"(a.im) : "s"(c), "v"(t2));
+  asm("v_fma_f64 %0, %1, %0, -%2" : "+v"(b.re) : "s"(c), "v"(ur));
+  asm("v_fma_f64 %0, %1, %0, -%2" : "+v"(b.im) : "s"(c), "v"(ui));
+}
+
+// FP32 (complex64 states): the same in-place form with v_fma_f32 / v_mul_f32.
+template <>
+__device__ __forceinline__ void rot_pair<float>(Cx<float>& a, Cx<float>& b, float c, float sr,
+                                                float si) {
+  float ur, ui, t1, t2;
+  asm("v_mul_f32 %0, %1, %2" : "=v"(ur) : "v"(sr), "v"(a.re));
+  asm("v_fma_f32 %0, %1, %2, %0" : "+v"(ur) : "v"(si), "v"(a.im));
+  asm("v_mul_f32 %0, %1, %2" : "=v"(ui) : "v"(sr), "v"(a.im));
+  asm("v_fma_f32 %0, -%1, %2, %0" : "+v"(ui) : "v"(si), "v"(a.re));
+  asm("v_mul_f32 %0, %1, %2" : "=v"(t1) : "v"(si), "v"(b.im));
+  asm("v_fma_f32 %0, %1, %2, -%0" : "+v"(t1) : "v"(sr), "v"(b.re));
+  asm("v_mul_f32 %0, %1, %2" : "=v"(t2) : "v"(si), "v"(b.re));
+  asm("v_fma_f32 %0, %1, %2, %0" : "+v"(t2) : "v"(sr), "v"(b.im));
+  asm("v_fma_f32 %0, %1, %0, %2" : "+v"(a.re) : "v"(c), "v"(t1));
+  asm("v_fma_f32 %0, %1, %0, %2" : "+v"(a.im) : "v"(c), "v"(t2));
+  asm("v_fma_f32 %0, %1, %0, -%2" : "+v"(b.re) : "v"(c), "v"(ur));
+  asm("v_fma_f32 %0, %1, %0, -%2" : "+v"(b.im) : "v"(c), "v"(ui));
+}
+
 template <typename T, int W2, int K2, int P>
 __device__ __forceinline__ void rot_sub(Cx<T>* v, T c, T sr, T si) {
   if constexpr (P + 1 < W2) {

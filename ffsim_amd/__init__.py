@@ -16,11 +16,21 @@ from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedH
 from ffsim_amd.protocols import apply_unitary, linear_operator
 from ffsim_amd.states import dim, dims, hartree_fock_state
 from ffsim_amd.trotter import simulate_trotter_double_factorized
-from ffsim_amd.variational import UCJOpSpinBalanced, UCJOpSpinless, UCJOpSpinUnbalanced
+from ffsim_amd.variational import (
+    GivensAnsatzOp,
+    NumNumAnsatzOpSpinBalanced,
+    UCJAnglesOpSpinBalanced,
+    UCJOpSpinBalanced,
+    UCJOpSpinless,
+    UCJOpSpinUnbalanced,
+)
 
 __all__ = [
     "DiagonalCoulombHamiltonian",
     "DoubleFactorizedHamiltonian",
+    "GivensAnsatzOp",
+    "NumNumAnsatzOpSpinBalanced",
+    "UCJAnglesOpSpinBalanced",
     "UCJOpSpinBalanced",
     "UCJOpSpinUnbalanced",
     "UCJOpSpinless",

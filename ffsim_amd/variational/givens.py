@@ -1,0 +1,172 @@
+"""Givens-rotation ansatz (python/ffsim/variational/givens.py).
+
+A sequence of Givens rotations with angles (theta, phi) on orbital pairs,
+then a layer of single-orbital phases; as an operator it is the orbital
+rotation ``to_orbital_rotation()``, applied by the GPU orbital-rotation
+path (givens.py:71-78).
+"""
+
+from __future__ import annotations
+
+import cmath
+import itertools
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from ffsim_amd import linalg as _linalg
+from ffsim_amd.gates.orbital_rotation import apply_orbital_rotation
+
+
+@dataclass(frozen=True)
+class GivensAnsatzOp:
+    """Givens rotations on ``interaction_pairs`` followed by single-orbital phases."""
+
+    norb: int
+    interaction_pairs: list
+    thetas: np.ndarray
+    phis: np.ndarray | None
+    phase_angles: np.ndarray | None
+
+    def __post_init__(self):
+        """Validation of givens.py:54-69."""
+        if len(self.thetas) != len(self.interaction_pairs):
+            raise ValueError(
+                "The number of thetas must equal the number of interaction pairs. "
+                f"Got {len(self.thetas)} and {len(self.interaction_pairs)}."
+            )
+        if self.phis is not None and len(self.phis) != len(self.interaction_pairs):
+            raise ValueError(
+                "The number of phis must equal the number of interaction pairs. "
+                f"Got {len(self.phis)} and {len(self.interaction_pairs)}."
+            )
+        if self.phase_angles is not None and len(self.phase_angles) != self.norb:
+            raise ValueError(
+                "The number of phase angles must equal the number of orbitals. "
+                f"Got {len(self.phase_angles)} and {self.norb}."
+            )
+
+    def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
+        return apply_orbital_rotation(vec, self.to_orbital_rotation(), norb, nelec, copy=copy)
+
+    @staticmethod
+    def n_params(norb: int, interaction_pairs, with_phis: bool = True,
+                 with_phase_angles: bool = True) -> int:
+        """givens.py:80-94."""
+        return (1 + with_phis) * len(interaction_pairs) + with_phase_angles * norb
+
+    def to_parameters(self) -> np.ndarray:
+        """thetas, then phis, then phase angles (the ones present)."""
+        parts = [np.asarray(self.thetas, dtype=float)]
+        if self.phis is not None:
+            parts.append(np.asarray(self.phis, dtype=float))
+        if self.phase_angles is not None:
+            parts.append(np.asarray(self.phase_angles, dtype=float))
+        return np.concatenate(parts)
+
+    @staticmethod
+    def from_parameters(params: np.ndarray, norb: int, interaction_pairs, with_phis: bool = True,
+                        with_phase_angles: bool = True) -> "GivensAnsatzOp":
+        """givens.py:107-149."""
+        n_pairs = len(interaction_pairs)
+        n_params = GivensAnsatzOp.n_params(norb, interaction_pairs, with_phis, with_phase_angles)
+        if len(params) != n_params:
+            raise ValueError(
+                "The number of parameters passed did not match the number expected "
+                "based on the function inputs. "
+                f"Expected {n_params} but got {len(params)}."
+            )
+        params = np.asarray(params, dtype=float)
+        thetas = params[:n_pairs]
+        rest = params[n_pairs:]
+        phis = rest[:n_pairs] if with_phis else None
+        phase_angles = rest[n_pairs:] if with_phis else rest
+        if not with_phase_angles:
+            phase_angles = None
+        return GivensAnsatzOp(norb=norb, interaction_pairs=interaction_pairs, thetas=thetas,
+                              phis=phis, phase_angles=phase_angles)
+
+    @staticmethod
+    def from_orbital_rotation(orbital_rotation: np.ndarray) -> "GivensAnsatzOp":
+        """Angles of the Givens decomposition, spread on a full brickwork pattern
+        (givens.py:151-176)."""
+        norb = orbital_rotation.shape[0]
+        rotations, phases = _linalg.givens_decomposition(orbital_rotation)
+        pairs, thetas, phis = [], [], []
+        for c, s, i, j in rotations:
+            pairs.append((i, j))
+            r, phi = cmath.polar(s)
+            thetas.append(math.atan2(r, c))
+            phis.append(phi)
+        pairs, thetas, phis = brickwork_givens_rotations(pairs, thetas, phis, norb)
+        return GivensAnsatzOp(norb=norb, interaction_pairs=pairs, thetas=np.array(thetas),
+                              phis=np.array(phis), phase_angles=np.angle(phases))
+
+    def to_orbital_rotation(self) -> np.ndarray:
+        """U = diag(e^{i phase}) then the rotations applied to column pairs (j, i) in
+        reverse order, each as (c x + s y, c y - conj(s) x) with c = cos theta,
+        s = sin(theta) e^{-i phi} (givens.py:178-196, BLAS zrot semantics)."""
+        n = len(self.interaction_pairs)
+        phis = np.zeros(n) if self.phis is None else np.asarray(self.phis, dtype=float)
+        angles = np.zeros(self.norb) if self.phase_angles is None else self.phase_angles
+        mat = np.diag(np.exp(1j * np.asarray(angles, dtype=float))).astype(complex)
+        for (i, j), theta, phi in zip(self.interaction_pairs[::-1], np.asarray(self.thetas)[::-1],
+                                      phis[::-1]):
+            c = math.cos(theta)
+            s = cmath.rect(math.sin(theta), -phi)
+            x, y = mat[:, j].copy(), mat[:, i].copy()
+            mat[:, j] = c * x + s * y
+            mat[:, i] = c * y - np.conj(s) * x
+        return mat
+
+    def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
+        if isinstance(other, GivensAnsatzOp):
+            if self.norb != other.norb or list(self.interaction_pairs) != list(other.interaction_pairs):
+                return False
+            if not np.allclose(self.thetas, other.thetas, rtol=rtol, atol=atol):
+                return False
+            for a, b in ((self.phis, other.phis), (self.phase_angles, other.phase_angles)):
+                if (a is None) != (b is None):
+                    return False
+                if a is not None and not np.allclose(a, b, rtol=rtol, atol=atol):
+                    return False
+            return True
+        return NotImplemented
+
+
+def brickwork_givens_rotations(pairs, thetas, phis, norb: int):
+    """Place rotations into a full brickwork of adjacent pairs (zero angles where
+    absent), keeping each rotation after every earlier one on its orbitals
+    (givens.py:228-296)."""
+    q, r = divmod(norb, 2)
+    even = [[((i, i + 1), 0.0, 0.0) for i in range(0, norb - 1, 2)] for _ in range(q + r)]
+    odd = [[((i, i + 1), 0.0, 0.0) for i in range(1, norb - 1, 2)] for _ in range(q)]
+    last_even = [-1] * norb
+    last_odd = [-1] * norb
+    for (i, j), theta, phi in zip(pairs, thetas, phis):
+        if i > j:
+            i, j, theta, phi = j, i, -theta, -phi
+        if i % 2 == 0:
+            k = max(last_even[i], last_even[j], last_odd[i], last_odd[j]) + 1
+            even[k][i // 2] = ((i, j), theta, phi)
+            last_even[i] = last_even[j] = k
+        else:
+            k = max(last_odd[i] + 1, last_odd[j] + 1, last_even[i], last_even[j])
+            odd[k][i // 2] = ((i, j), theta, phi)
+            last_odd[i] = last_odd[j] = k
+    out_pairs, out_thetas, out_phis = [], [], []
+    for ev, od in itertools.zip_longest(even, odd, fillvalue=()):
+        for layer in (ev, od):
+            for pair, theta, phi in layer:
+                out_pairs.append(pair)
+                out_thetas.append(theta)
+                out_phis.append(phi)
+    return out_pairs, out_thetas, out_phis
+
+
+def brickwork(norb: int, n_layers: int):
+    """Adjacent pairs of n_layers alternating brickwork layers (ucj_angles_spin_balanced.py:247)."""
+    for layer in range(n_layers):
+        for j in range(layer % 2, norb - 1, 2):
+            yield (j, j + 1)

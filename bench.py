@@ -160,6 +160,81 @@ def trotter_leg(norb: int = 18, n_factors: int = 10, time_step: float = 0.1, see
     return out
 
 
+def gradient_leg(seed: int = 12) -> dict:
+    """Energy gradient of LUCJ n_reps=4 at norb=12, nelec=(6,6) (668 parameters,
+    U(-pi, pi)), Hartree-Fock reference, synthetic H_DF with 8 factors: the
+    adjoint sweep's wall time against central finite differences, whose cost is
+    1336 energy evaluations (timed as one evaluation on the GPU and one on the
+    CPU oracle, scaled)."""
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd.variational.gradient import UCJLayout, _apply_forward, ucj_energy_and_gradient
+
+    norb, nelec, n_reps = 12, (6, 6), 4
+    pairs = lucj_pairs(norb)
+    theta = np.random.default_rng(seed).uniform(-np.pi, np.pi, 668)
+    ham = ffsim_amd.random.random_df_hamiltonian_normal(norb, n_factors=8, seed=7)
+    linop = ffsim_amd.linear_operator(ham, norb, nelec)
+    psi0 = torch.from_numpy(ffsim_amd.hartree_fock_state(norb, nelec)).cuda()
+    layout = UCJLayout.make(norb, n_reps, pairs)
+
+    def energy():
+        psi = psi0.clone()
+        _apply_forward(layout, theta, psi, nelec)
+        return float(torch.vdot(psi, linop.matvec_device(psi)).real.item())
+
+    for _ in range(2):
+        ucj_energy_and_gradient(theta, linop, psi0, norb=norb, nelec=nelec, n_reps=n_reps,
+                                interaction_pairs=pairs)
+        energy()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ucj_energy_and_gradient(theta, linop, psi0, norb=norb, nelec=nelec, n_reps=n_reps,
+                                interaction_pairs=pairs)
+    t_grad = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        energy()
+    t_energy = (time.perf_counter() - t0) / reps
+    return {
+        "workload": "LUCJ n_reps=4 norb=12 (6,6), 668 params, HF reference, H_DF 8 factors",
+        "adjoint_gradient_ms": round(t_grad * 1e3, 2),
+        "gpu_energy_ms": round(t_energy * 1e3, 3),
+        "gpu_central_fd_ms_estimated": round(1336 * t_energy * 1e3, 1),
+        "adjoint_speedup_vs_gpu_fd": round(1336 * t_energy / t_grad, 1),
+    }
+
+
+def cpu_baseline_gradient(threads: int, seed: int = 12) -> dict:
+    """CPU side of the gradient workload: one energy evaluation of the oracle's
+    restatement of ffsim's CPU path, scaled to a central-FD gradient (1336
+    evaluations, ffsim's gradient mechanism)."""
+    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
+    import ffsim_amd
+    from oracle import oracle as O
+
+    O.build()
+    norb, nelec, n_reps = 12, (6, 6), 4
+    pairs = lucj_pairs(norb)
+    theta = np.random.default_rng(seed).uniform(-np.pi, np.pi, 668)
+    ham = ffsim_amd.random.random_df_hamiltonian_normal(norb, n_factors=8, seed=7)
+    hf = ffsim_amd.hartree_fock_state(norb, nelec)
+
+    def h(v):
+        return O.df_hamiltonian_matvec(v, ham.one_body_tensor, ham.diag_coulomb_mats,
+                                       ham.orbital_rotations, ham.constant, norb, nelec, False)
+
+    O.ucj_energy(theta, h, hf, norb, nelec, n_reps, pairs)
+    t0 = time.perf_counter()
+    O.ucj_energy(theta, h, hf, norb, nelec, n_reps, pairs)
+    t_energy = time.perf_counter() - t0
+    return {"energy_ms": round(t_energy * 1e3, 1),
+            "central_fd_gradient_s_estimated": round(1336 * t_energy, 1)}
+
+
 def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
     """Served mixed stream vs the unbatched plain path, same Poisson arrivals.
 
@@ -263,6 +338,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-trotter", action="store_true")
+    ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--serving-load", type=float, default=0.9)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
@@ -385,12 +461,18 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("trotter leg")
         result["trotter_norb18"] = trotter_leg()
+    if not args.no_gradient:
+        log("gradient leg")
+        result["gradient_norb12"] = gradient_leg()
+        log(f"gradient: {result['gradient_norb12']}")
     if not args.no_serving:
         log("serving leg")
         result["serving"] = serving_leg(args.serving_requests, args.serving_load)
         log(f"serving: {result['serving']}")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, args.cpu_threads)
+        if not args.no_gradient:
+            result["cpu_baseline"]["gradient_norb12"] = cpu_baseline_gradient(args.cpu_threads)
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
     print(json.dumps(result), flush=True)
     return 0

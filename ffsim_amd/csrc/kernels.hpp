@@ -331,21 +331,37 @@ __global__ void __launch_bounds__(NTHR)
     }
   };
   // issue every global load of tile (nr rows, batch offset b0, nb columns) into buf
-  auto load_regs = [&](Cx<T>* buf, int nr, int b0, int nb) {
+  // Row addresses come from `rt`: the staged table in LDS (next tile in the same
+  // fiber) or the global row table of the next fiber.  The two are separate
+  // instantiations: a pointer that may be either would compile to flat loads,
+  // and waiting for those (vmcnt) would also drain the stores in flight.
+  // All row addresses are read first and the element loads issued after them, so
+  // the global-table case waits once for its addresses instead of once per row.
+  auto load_regs = [&](Cx<T>* buf, int nr, int b0, int nb, const int32_t* rt) {
+    int ra[CH];
     if (MODE == MODE_ROWS) {
       const int j = tid % W, i0 = tid / W;
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const int i = i0 + k * RSTEP;
+        ra[k] = (i < nr && j < nb) ? rt[i] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
         buf[k] = Cx<T>{T(0), T(0)};
-        if (i < nr && j < nb) buf[k] = src[(long)rows[i] * ld_src + b0 + j];
+        if (ra[k] >= 0) buf[k] = src[(long)ra[k] * ld_src + b0 + j];
       }
     } else {
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const int i = tid + (k / W) * NTHR, j = k % W;
+        ra[k] = (i < nr && j < nb) ? rt[i] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int j = k % W;
         buf[k] = Cx<T>{T(0), T(0)};
-        if (i < nr && j < nb) buf[k] = src[(long)(b0 + j) * ld_src + rows[i]];
+        if (ra[k] >= 0) buf[k] = src[(long)(b0 + j) * ld_src + ra[k]];
       }
     }
   };
@@ -460,13 +476,22 @@ __global__ void __launch_bounds__(NTHR)
     const int nb = min(W, n_batch - b0);
     const long tn = t + gridDim.x;
     const int fn = tn < n_tiles ? (int)(tn / n_bt) : -1;
-    const bool prefetch = (fn == f);
+    const bool same_fiber = (fn == f);
+    // the next tile is prefetched also when it starts a new fiber: its rows then
+    // come from the global row table (one more dependent load, hidden by the
+    // sub-passes below) instead of the staged table
+    const Fiber fnext = fibers[fn >= 0 ? fn : f];  // a uniform (scalar) load
+    const Fiber fdn = same_fiber ? fd : fnext;
+    const int nrn = fn >= 0 ? fdn.n_rows : 0;
     const int b0n = (int)(tn % n_bt) * W;
-    const int nbn = prefetch ? min(W, n_batch - b0n) : 0;
-    // unconditional (no loads when not prefetching) so that buf is never carried
+    const int nbn = fn >= 0 ? min(W, n_batch - b0n) : 0;
+    // unconditional (no loads after the last tile) so that buf is never carried
     // around the loop: a loop-carried buf makes the compiler copy it at the back
     // edge behind a vmcnt(0) that would also drain this tile's stores
-    load_regs(buf, prefetch ? nr : 0, b0n, nbn);
+    if (same_fiber)
+      load_regs(buf, nrn, b0n, nbn, rows);
+    else
+      load_regs(buf, nrn, b0n, nbn, row_table + fdn.row_off);
 
     // ---- register sub-passes on the current tile
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
@@ -496,16 +521,14 @@ __global__ void __launch_bounds__(NTHR)
     store_tile(nr, b0, nb);
     __syncthreads();
     if (tn >= n_tiles) break;
-    if (prefetch) {
-      regs_to_lds(buf, nr);
-      load_rows(kCovered, nr, b0n, nbn);
-    } else {
+    if (!same_fiber) {
       f = fn;
-      fd = fibers[f];
+      fd = fdn;
       stage_fiber(fd);
       __syncthreads();
-      load_rows(0, fd.n_rows, b0n, min(W, n_batch - b0n));
     }
+    regs_to_lds(buf, nrn);
+    load_rows(kCovered, nrn, b0n, nbn);
     t = tn;
   }
 }

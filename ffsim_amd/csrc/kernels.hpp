@@ -289,18 +289,20 @@ __global__ void __launch_bounds__(NTHR)
 
 
   // stage the row addresses and group tables of fiber f
-  auto stage_fiber = [&](const Fiber& fd) {
+  // The group tables and wave schedule depend only on the fiber's electron count
+  // in the window (kp); `tables` is false when the previous fiber had the same kp.
+  auto stage_fiber = [&](const Fiber& fd, bool tables) {
     for (int i = tid; i < fd.n_rows; i += NTHR) rows[i] = row_table[fd.row_off + i];
     if (diag) {
       for (int i = tid; i < fd.n_rows; i += NTHR) {
-        const uint64_t sv = strings[row_table[fd.row_off + i]];
+        const uint64_t sv = strings[rows[i]];
         if (narrow_str)
           rstr32[i] = (uint32_t)sv;
         else
           rstr64[i] = sv;
       }
     }
-    if (n_sub > 0) {
+    if (n_sub > 0 && tables) {
       int d_base = 0, m_base = 0;
       for (int u = 0; u < los.n; ++u) {
         const int* ix = ga.index + (fd.kp * ga.n_lo2 + los.lo2[u]) * 4;
@@ -462,7 +464,7 @@ __global__ void __launch_bounds__(NTHR)
     for (int k = tid; k < 2 * n_diag; k += NTHR) dph[k] = diag[k];
   int f = (int)(t / n_bt);
   Fiber fd = fibers[f];
-  stage_fiber(fd);
+  stage_fiber(fd, true);
   __syncthreads();
   Cx<T> buf[CH];
   {
@@ -523,8 +525,9 @@ __global__ void __launch_bounds__(NTHR)
     if (tn >= n_tiles) break;
     if (!same_fiber) {
       f = fn;
+      const bool tables = fdn.kp != fd.kp;
       fd = fdn;
-      stage_fiber(fd);
+      stage_fiber(fd, tables);
       __syncthreads();
     }
     regs_to_lds(buf, nrn);

@@ -625,6 +625,12 @@ __global__ void __launch_bounds__(256)
   for (int c = threadIdx.x; c < n_cols; c += blockDim.x) {
     const uint64_t cb = col_str[c];
     const double g = gamma ? gamma[c] : 0.0;
+    // all loads of this column first: src may alias dst (in-place evolution), so
+    // a load placed after the previous row's store could not be issued early
+    Cx<T> vin[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (r0 + r < n_rows) vin[r] = src[(long)(r0 + r) * ld + c];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (r0 + r >= n_rows) break;
@@ -640,7 +646,7 @@ __global__ void __launch_bounds__(256)
       }
       const double E = rho[r] + g + cross;
       const long idx = (long)(r0 + r) * ld + c;
-      const Cx<T> v = src[idx];
+      const Cx<T> v = vin[r];
       Cx<T> o;
       if (MODE == DIAG_EVOLVE) {
         double sn, cs;

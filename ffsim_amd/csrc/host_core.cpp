@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 
 namespace ffsim_amd {
@@ -288,6 +290,109 @@ std::vector<PlannedPass> plan_greedy_max(const std::vector<KStep>& steps, int n,
   return out;
 }
 
+namespace {
+// Rotations of sub-window [lo, lo + w2) that can run now: in order, those not
+// yet applied whose orbitals no earlier pending rotation outside the window has
+// touched.
+std::vector<size_t> sub_window_closure(const std::vector<KStep>& steps,
+                                       const std::vector<char>& done, int lo, int w2, int n) {
+  std::vector<size_t> cur;
+  std::vector<char> blocked(n + 1, 0);
+  for (size_t r = 0; r < steps.size(); ++r) {
+    if (done[r]) continue;
+    const int p = steps[r].p;
+    if (p >= lo && p + 1 < lo + w2 && !blocked[p] && !blocked[p + 1])
+      cur.push_back(r);
+    else
+      blocked[p] = blocked[p + 1] = 1;
+  }
+  return cur;
+}
+
+// Beam search over sequences of sub-window starts for the fewest sub-passes
+// (each step takes the whole closure of its sub-window).  Returns the starts.
+std::vector<int> plan_sub_window_beam(const std::vector<KStep>& steps, int n, int w2,
+                                      size_t width) {
+  struct Node {
+    std::vector<char> done;
+    size_t n_done;
+    std::vector<int> starts;
+  };
+  std::vector<Node> beam{{std::vector<char>(steps.size(), 0), 0, {}}};
+  while (true) {
+    for (const Node& b : beam)
+      if (b.n_done == steps.size()) return b.starts;
+    std::vector<Node> next;
+    for (const Node& b : beam) {
+      for (int lo = 0; lo + w2 <= n; ++lo) {
+        std::vector<size_t> c = sub_window_closure(steps, b.done, lo, w2, n);
+        if (c.empty()) continue;
+        Node x = b;
+        for (size_t r : c) x.done[r] = 1;
+        x.n_done += c.size();
+        x.starts.push_back(lo);
+        next.push_back(std::move(x));
+      }
+    }
+    if (next.empty()) throw std::runtime_error("sub-pass planner made no progress");
+    std::stable_sort(next.begin(), next.end(),
+                     [](const Node& a, const Node& b) { return a.n_done > b.n_done; });
+    beam.clear();
+    for (Node& x : next) {
+      bool dup = false;
+      for (const Node& u : beam)
+        if (u.done == x.done) {
+          dup = true;
+          break;
+        }
+      if (!dup) beam.push_back(std::move(x));
+      if (beam.size() >= width) break;
+    }
+  }
+}
+
+std::vector<PlannedPass> passes_from_starts(const std::vector<KStep>& steps, int n, int w2,
+                                            const std::vector<int>& starts) {
+  std::vector<PlannedPass> out;
+  std::vector<char> done(steps.size(), 0);
+  for (int lo : starts) {
+    PlannedPass P;
+    P.lo = lo;
+    for (size_t r : sub_window_closure(steps, done, lo, w2, n)) {
+      KStep k = steps[r];
+      k.p -= lo;
+      P.steps.push_back(k);
+      done[r] = 1;
+    }
+    out.push_back(P);
+  }
+  return out;
+}
+
+// The schedule depends only on the sequence of pair positions, which is the
+// same for every unitary of a size (Clements order), so searches are cached.
+std::mutex g_beam_mu;
+std::map<std::vector<int>, std::vector<int>> g_beam_cache;
+
+std::vector<int> cached_beam_starts(const std::vector<KStep>& steps, int n, int w2) {
+  std::vector<int> key;
+  key.reserve(steps.size() + 2);
+  key.push_back(n);
+  key.push_back(w2);
+  for (const KStep& k : steps) key.push_back(k.p);
+  {
+    std::lock_guard<std::mutex> lk(g_beam_mu);
+    auto it = g_beam_cache.find(key);
+    if (it != g_beam_cache.end()) return it->second;
+  }
+  std::vector<int> starts = plan_sub_window_beam(steps, n, w2, 32);
+  std::lock_guard<std::mutex> lk(g_beam_mu);
+  if (g_beam_cache.size() > 4096) g_beam_cache.clear();
+  g_beam_cache[key] = starts;
+  return starts;
+}
+}  // namespace
+
 void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
                      std::vector<SubPass>& subs, std::vector<LayerPair>& layers) {
   subs.clear();
@@ -299,6 +404,11 @@ void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
   std::vector<PlannedPass> sp = plan_passes(window_steps, w, w2, false, -1);
   {
     std::vector<PlannedPass> alt = plan_greedy_max(window_steps, w, w2);
+    if (alt.size() < sp.size()) sp.swap(alt);
+  }
+  if (w2 < w && sp.size() > 2) {
+    std::vector<PlannedPass> alt =
+        passes_from_starts(window_steps, w, w2, cached_beam_starts(window_steps, w, w2));
     if (alt.size() < sp.size()) sp.swap(alt);
   }
   for (const PlannedPass& P : sp) {

@@ -42,16 +42,20 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> dict:
+def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int,
+                 repeats: int = 3, single_thread_rotations: int = 12) -> dict:
     """Time a bounded sample of the same workload with the CPU oracle.
 
     Sample: one alpha-sector orbital rotation (the 120 Givens rotations and 16
     phase shifts of ffsim's schedule), one transpose copy and one diag-Coulomb
     evolution on the norb=16 state.  An application costs 10 sector rotations,
     10 transpose copies and 4 diag-Coulomb passes on that schedule, so
-    apps/s = 1 / (10 T_rot + 10 T_transpose + 4 T_dc).
+    apps/s = 1 / (10 T_rot + 10 T_transpose + 4 T_dc).  With ``threads``
+    threads: one warm-up sample, then the median of ``repeats`` samples.  On one
+    thread: the first ``single_thread_rotations`` Givens rotations (scaled to
+    the sector rotation by rotation count), one transpose and one diag-Coulomb
+    pass.
     """
-    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
     from oracle import oracle as O
 
     O.build()
@@ -60,22 +64,43 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> di
     v = np.ascontiguousarray(vec_host.reshape(da, db))
     U = op.orbital_rotations[0]
     rots, phases = O.givens_decomposition(U)
-    t0 = time.perf_counter()
-    for c, s, i, j in rots:
-        O._apply_adjacent(v, c, s.conjugate(), (i, j), norb, nelec[0])
-    for i, ph in enumerate(phases):
-        O._phase_shift_in_place(v, complex(ph), O.one_subspace_indices(norb, nelec[0], (i,)))
-    t_rot = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    vt = O._transpose_copy(v)
-    t_tr = time.perf_counter() - t0
-    del vt
     j_aa, j_ab = op.diag_coulomb_mats[0]
-    t0 = time.perf_counter()
-    O.apply_diag_coulomb_evolution(v.reshape(-1), (j_aa, j_ab, j_aa), -1.0, norb, nelec, copy=False)
-    t_dc = time.perf_counter() - t0
+    phase_idx = [O.one_subspace_indices(norb, nelec[0], (i,)) for i in range(norb)]
+
+    def sample(n_rot=None):
+        sel = rots if n_rot is None else rots[:n_rot]
+        t0 = time.perf_counter()
+        for c, s_, i, j in sel:
+            O._apply_adjacent(v, c, s_.conjugate(), (i, j), norb, nelec[0])
+        if n_rot is None:
+            for i, ph in enumerate(phases):
+                O._phase_shift_in_place(v, complex(ph), phase_idx[i])
+        t_rot = time.perf_counter() - t0
+        if n_rot is not None:
+            t_rot *= len(rots) / max(1, len(sel))
+        t0 = time.perf_counter()
+        vt = O._transpose_copy(v)
+        t_tr = time.perf_counter() - t0
+        del vt
+        t0 = time.perf_counter()
+        O.apply_diag_coulomb_evolution(v.reshape(-1), (j_aa, j_ab, j_aa), -1.0, norb, nelec,
+                                       copy=False)
+        t_dc = time.perf_counter() - t0
+        return t_rot, t_tr, t_dc
+
+    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
+    sample()  # warm-up
+    runs = [sample() for _ in range(repeats)]
+    t_rot, t_tr, t_dc = (float(np.median([r[k] for r in runs])) for k in range(3))
     t_app = 10 * t_rot + 10 * t_tr + 4 * t_dc
-    log(f"cpu sample: sector rotation {t_rot:.2f}s transpose {t_tr:.2f}s diag-coulomb {t_dc:.2f}s")
+    log(f"cpu sample ({threads} threads, median of {repeats}): sector rotation {t_rot:.2f}s "
+        f"transpose {t_tr:.2f}s diag-coulomb {t_dc:.2f}s")
+    os.environ["FFSIM_ORACLE_THREADS"] = "1"
+    r1, tr1, dc1 = sample(single_thread_rotations)
+    t_app1 = 10 * r1 + 10 * tr1 + 4 * dc1
+    log(f"cpu sample (1 thread): sector rotation {r1:.2f}s (from {single_thread_rotations} "
+        f"rotations) transpose {tr1:.2f}s diag-coulomb {dc1:.2f}s")
+    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
     return {
         "value": 1.0 / t_app,
         "unit": "LUCJ applications/s",
@@ -83,11 +108,18 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int) -> di
         "kind": "port",
         "sample": (
             f"C restatement of ffsim's Rust kernels with ffsim's schedule (one FFI call per Givens "
-            f"rotation, per-call threads, transpose copies), {threads} threads: one alpha-sector "
-            f"orbital rotation ({t_rot:.2f} s), one transpose copy ({t_tr:.2f} s) and one "
-            f"diag-Coulomb evolution ({t_dc:.2f} s) of the norb=16 (8,8) state, scaled to one "
-            f"n_reps=4 application = 10 rotations + 10 transposes + 4 diag-Coulomb passes"
+            f"rotation, per-call threads, transpose copies), {threads} threads, one warm-up then "
+            f"the median of {repeats}: one alpha-sector orbital rotation ({t_rot:.2f} s), one "
+            f"transpose copy ({t_tr:.2f} s) and one diag-Coulomb evolution ({t_dc:.2f} s) of the "
+            f"norb=16 (8,8) state, scaled to one n_reps=4 application = 10 rotations + 10 "
+            f"transposes + 4 diag-Coulomb passes"
         ),
+        "single_thread": {
+            "value": 1.0 / t_app1,
+            "sample": (f"1 thread: {single_thread_rotations} of the {len(rots)} Givens rotations "
+                       f"scaled by rotation count ({r1:.2f} s per sector rotation), one transpose "
+                       f"({tr1:.2f} s), one diag-Coulomb pass ({dc1:.2f} s)"),
+        },
     }
 
 
@@ -126,7 +158,7 @@ def trotter_leg(norb: int = 18, n_factors: int = 10, time_step: float = 0.1, see
     results = {}
     for dtype in (torch.complex128, torch.complex64):
         tag = "c128" if dtype == torch.complex128 else "c64"
-        psi = psi0.to(dtype)
+        psi = psi0.to(dtype, copy=True)  # both runs start from the same state
         if dtype == torch.complex64:
             del psi0
         torch.cuda.synchronize()
@@ -152,9 +184,16 @@ def trotter_leg(norb: int = 18, n_factors: int = 10, time_step: float = 0.1, see
     psi128, psi64 = results["c128"], results["c64"].to(torch.complex128)
     out["c64_vs_c128_max_abs"] = float((psi64 - psi128).abs().max().item())
     chunk = 1 << 30  # torch.vdot takes at most 2^31 - 1 elements
-    overlap = sum(torch.vdot(psi128[i : i + chunk], psi64[i : i + chunk]).item()
-                  for i in range(0, dim, chunk))
-    out["c64_vs_c128_infidelity"] = float(1.0 - abs(overlap))
+
+    def dot(x, y):
+        return sum(torch.vdot(x[i : i + chunk], y[i : i + chunk]).item()
+                   for i in range(0, dim, chunk))
+
+    overlap, n128, n64 = dot(psi128, psi64), dot(psi128, psi128).real, dot(psi64, psi64).real
+    # 1 - |<a|b>|^2 / (<a|a><b|b>): the norm drift of the c64 run is reported apart
+    out["c64_vs_c128_infidelity"] = float(1.0 - abs(overlap) ** 2 / (n128 * n64))
+    out["c64_norm_error"] = float(abs(math.sqrt(n64) - 1.0))
+    out["c128_norm_error"] = float(abs(math.sqrt(n128) - 1.0))
     del results, psi128, psi64
     torch.cuda.empty_cache()
     return out
@@ -304,6 +343,19 @@ def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
         int(not np.array_equal(t.result(), plain(r))) for t, r in zip(tickets[:32], reqs[:32])
     )
 
+    # saturation: every request queued at once, plain path one after another
+    start = time.perf_counter()
+    for r in reqs:
+        plain(r)
+    burst_plain = n_requests / (time.perf_counter() - start)
+    burst = LUCJServer(hbm_budget=budget, slots_per_stream=2, n_streams=4, poll_interval=0.0)
+    start = time.perf_counter()
+    burst_tickets = [burst.submit(r) for r in reqs]
+    burst.drain(timeout=600)
+    burst_served = n_requests / (time.perf_counter() - start)
+    if not all(t.done() for t in burst_tickets):
+        raise RuntimeError("saturation run left requests unfinished")
+
     def summary(lat, span):
         lat = np.asarray(lat)
         return {
@@ -324,6 +376,8 @@ def serving_leg(n_requests: int, load: float, seed: int = 99) -> dict:
         "hbm_budget_bytes": budget,
         "peak_hbm_bytes": server.peak_bytes,
         "served_vs_alone_bitwise_mismatches_of_32": mismatches,
+        "saturation_states_per_s": {"served": round(burst_served, 1),
+                                    "plain": round(burst_plain, 1)},
     }
 
 

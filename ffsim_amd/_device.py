@@ -58,6 +58,50 @@ def host_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
+# Copies between pageable host memory and the GPU go through the runtime's
+# small staging buffers (about 4 GB/s measured for a 188 MB state); through a
+# pinned bounce buffer they run at PCIe rate.  States below this size are
+# copied directly.  Pinned buffers come from torch's caching host allocator,
+# so the pinned footprint stays at the largest state copied.
+_STAGE_MIN_BYTES = 1 << 20
+
+
+def host_to_device(arr: np.ndarray, dev: torch.device) -> torch.Tensor:
+    """Copy a C-contiguous 1-D host array to a new device tensor (current stream)."""
+    src = torch.from_numpy(arr)
+    if arr.nbytes < _STAGE_MIN_BYTES:
+        return src.to(dev)
+    staged = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+    staged.copy_(src)
+    # the caching host allocator keeps `staged` until the copy has run
+    return staged.to(dev, non_blocking=True)
+
+
+def start_device_to_host(t: torch.Tensor) -> torch.Tensor:
+    """Enqueue the copy of a device tensor into pinned host memory on the current
+    stream; the result is valid once the stream has reached this point."""
+    staged = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    staged.copy_(t, non_blocking=True)
+    return staged
+
+
+def finish_device_to_host(staged: torch.Tensor) -> np.ndarray:
+    """Copy a pinned host tensor (its device-to-host copy complete) into a new
+    numpy array, so that the pinned buffer returns to the allocator."""
+    out = np.empty(tuple(staged.shape), dtype=staged.numpy().dtype)
+    torch.from_numpy(out).copy_(staged)
+    return out
+
+
+def device_to_host(t: torch.Tensor) -> np.ndarray:
+    """Synchronous copy of a device tensor into a new numpy array."""
+    if t.numel() * t.element_size() < _STAGE_MIN_BYTES:
+        return t.cpu().numpy()
+    staged = start_device_to_host(t)
+    torch.cuda.current_stream().synchronize()
+    return finish_device_to_host(staged)
+
+
 @dataclass
 class DeviceState:
     """A working 1-D device tensor plus the conversion back to the caller's type."""
@@ -83,8 +127,8 @@ def to_device(vec, copy: bool) -> DeviceState:
     arr = np.asarray(vec)
     if arr.dtype not in (np.complex128, np.complex64):
         arr = arr.astype(np.complex128)
-    t = torch.from_numpy(np.ascontiguousarray(arr).reshape(-1)).to(dev)
-    return DeviceState(t, lambda out: out.cpu().numpy())
+    t = host_to_device(np.ascontiguousarray(arr).reshape(-1), dev)
+    return DeviceState(t, device_to_host)
 
 
 def dims(norb: int, nelec) -> tuple[int, int]:

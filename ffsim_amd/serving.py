@@ -115,6 +115,7 @@ class DeviceBackend:
         from ffsim_amd import _device
 
         self.torch = torch
+        self._device = _device
         self.device = _device.require_device()
         # default priority only; at most the number of hardware queues
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]
@@ -135,20 +136,23 @@ class DeviceBackend:
         host = torch.from_numpy(np.ascontiguousarray(host, dtype=np.complex128)).to(cdt)
         stream = self.streams[stream_index]
         with torch.cuda.stream(stream):
-            t = host.pin_memory().to(self.device, non_blocking=True)
+            t = self._device.host_to_device(host.numpy(), self.device)
             req.op.apply_in_place(t, req.norb, tuple(req.nelec))
+            # the result's copy to pinned host memory is queued behind the kernels,
+            # so a finished request needs only a host-side copy to be collected
+            staged = self._device.start_device_to_host(t) if req.to_host else None
             ev = torch.cuda.Event()
             ev.record(stream)
-        return (t, ev, stream)
+        return (t, ev, stream, staged)
 
     def done(self, handle) -> bool:
         return handle[1].query()
 
     def collect(self, req: LUCJRequest, handle):
-        t, ev, stream = handle
+        t, ev, stream, staged = handle
         ev.synchronize()
-        if req.to_host:
-            return t.cpu().numpy()
+        if staged is not None:
+            return self._device.finish_device_to_host(staged)
         return t
 
     def release(self, handle) -> None:

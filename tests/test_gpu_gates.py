@@ -279,7 +279,8 @@ def test_resident_tensor_in_place(rng):
                                atol=TOL, rtol=0)
 
 
-@pytest.mark.parametrize("norb,nelec", [(8, (4, 4)), (12, (6, 6)), (14, (7, 6))])
+@pytest.mark.parametrize("norb,nelec", [(8, (4, 4)), (12, (6, 6)), (13, (6, 5)), (14, (7, 6)),
+                                        (14, (7, 7))])
 def test_complex64_error_vs_fp64_oracle(norb, nelec, rng):
     """complex64 storage: the error against the complex128 oracle is stated
     per result and must stay at single-precision level (5e-5 absolute)."""

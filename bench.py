@@ -123,6 +123,21 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int,
     }
 
 
+def copy_bandwidth(src, dst, reps: int = 10) -> float:
+    """Device-to-device copy rate in GB/s (bytes read + bytes written), the
+    practical HBM ceiling on this GPU for a streaming read-write kernel."""
+    import torch
+
+    dst.copy_(src)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    end.record()
+    end.synchronize()
+    return 2.0 * src.numel() * src.element_size() * reps / (start.elapsed_time(end) * 1e6)
+
+
 def measured_traffic(kernel: str, workload: str, dtype: str):
     """HBM bytes per launch of ``kernel`` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), or None."""
@@ -426,7 +441,8 @@ def main() -> int:
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"copy probe: 3 copies of {n} bytes (read {n} + write {n} per copy)")
+        log(f"copy probe: 3 copies of {n} bytes (read {n} + write {n} per copy); "
+            f"{copy_bandwidth(src, dst):.1f} GB/s")
         return 0
     norb = args.norb
     nelec = (norb // 2, norb // 2)
@@ -479,6 +495,14 @@ def main() -> int:
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "launches_per_step": fiber_n / args.steps,
     }
+    # practical ceiling: a plain device copy of 1 GiB on the same GPU, after the timed region
+    probe_src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+    probe_dst = torch.empty_like(probe_src)
+    copy_gbs = copy_bandwidth(probe_src, probe_dst)
+    del probe_src, probe_dst
+    roofline["copy_GBs"] = round(copy_gbs, 1)
+    roofline["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    log(f"device copy: {copy_gbs:.1f} GB/s")
     tr = measured_traffic("fiber_kernel", f"lucj_norb{norb}_nreps{args.n_reps}", args.dtype)
     if tr is not None:
         roofline["traffic"] = tr["bytes_per_launch"]

@@ -70,7 +70,8 @@ int guarded(F&& f) {
 // ------------------------------------------------------------- device state
 __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ col_str,
                                      const DiagHeader* __restrict__ hdr,
-                                     const double* __restrict__ coef, double* __restrict__ gamma) {
+                                     const double* __restrict__ coef, double* __restrict__ gamma,
+                                     int evolve) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cols) return;
   const int n = hdr->norb;
@@ -97,7 +98,14 @@ __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ co
     for (int j = 0; j < n; ++j)
       if ((bits >> j) & 1) acc += lin_col[j];
   }
-  gamma[c] = acc;
+  if (evolve) {
+    double sn, cs;
+    sincos(-hdr->scale * acc, &sn, &cs);
+    gamma[2 * c] = cs;
+    gamma[2 * c + 1] = sn;
+  } else {
+    gamma[c] = acc;
+  }
 }
 
 namespace {
@@ -728,23 +736,29 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   put(spec.lin_row, 3 * nn, norb);
   put(spec.lin_col, 3 * nn + norb, norb);
   const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
-  const size_t goff = prog.append(nullptr, (size_t)n_cols * sizeof(double));
+  // per column: gamma (contractions) or its phase factor (cos, sin) (evolution)
+  const int evolve = op == DIAG_EVOLVE ? 1 : 0;
+  const size_t goff = prog.append(nullptr, (size_t)n_cols * (evolve ? 2 : 1) * sizeof(double));
   prog.commit();
   const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
   const double* dc = prog.dev<double>(coff);
   double* dg = prog.dev<double>(goff);
   hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((n_cols + 255) / 256)), dim3(256), 0,
-                     stream, (int)n_cols, cs.d, dh, dc, dg);
+                     stream, (int)n_cols, cs.d, dh, dc, dg, evolve);
   HIP_CHECK(hipGetLastError());
-  constexpr int R = 4;
-  const unsigned grid = (unsigned)((n_rows + R - 1) / R);
+  // blocks of R rows x up to kColsPerBlock columns (many blocks for a full GPU)
+  constexpr int R = 8;
+  constexpr int kColsPerBlock = 2048;
+  const int cols_blk = (int)std::min<long>(n_cols, kColsPerBlock);
+  const dim3 grid((unsigned)((n_rows + R - 1) / R), (unsigned)((n_cols + cols_blk - 1) / cols_blk));
+  require(grid.y <= 65535, FFSIM_ERR_VALUE, "too many columns");
   const double moved = (double)elem_bytes(dtype) * (double)n_rows * (double)n_cols *
                        (op == DIAG_CONTRACT_ACC ? 3.0 : 2.0);
   const long pk = prof_start(PROF_DIAG, moved, stream);
 #define FFSIM_DIAG_LAUNCH(TT, MM)                                                           \
-  hipLaunchKernelGGL((diag_kernel<TT, R, MM>), dim3(grid), dim3(256), 0, stream,            \
+  hipLaunchKernelGGL((diag_kernel<TT, R, MM>), grid, dim3(256), 0, stream,                  \
                      reinterpret_cast<const Cx<TT>*>(src), reinterpret_cast<Cx<TT>*>(dst), \
-                     n_cols, (int)n_rows, (int)n_cols, rs.d, cs.d, dg, dh, dc)
+                     n_cols, (int)n_rows, (int)n_cols, cols_blk, rs.d, cs.d, dg, dh, dc)
   if (dtype == FFSIM_C128) {
     if (op == DIAG_EVOLVE) FFSIM_DIAG_LAUNCH(double, DIAG_EVOLVE);
     else if (op == DIAG_CONTRACT) FFSIM_DIAG_LAUNCH(double, DIAG_CONTRACT);

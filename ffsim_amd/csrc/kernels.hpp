@@ -551,129 +551,218 @@ struct DiagHeader {
 enum { DIAG_EVOLVE = 0, DIAG_CONTRACT = 1, DIAG_CONTRACT_ACC = 2 };
 enum { DF_QUAD = 1, DF_CROSS = 2, DF_LIN = 4 };
 
-// Column term gamma[c] (quadratic + linear) for every column string.
+// Column term gamma[c] (quadratic + linear) for every column string.  For
+// evolution it writes the column's phase factor exp(-i scale gamma[c]) as
+// (cos, sin) pairs instead.
 __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ col_str,
                                      const DiagHeader* __restrict__ hdr,
-                                     const double* __restrict__ coef, double* __restrict__ gamma);
+                                     const double* __restrict__ coef, double* __restrict__ gamma,
+                                     int evolve);
 
+// One pass of a diagonal operator over an R-row x n_cols_blk-column block of
+// the state.  E(a, b) = rho_a + gamma_b + cross(a, b), cross = sum over the
+// occupied orbitals q of b of w_a[q] (number representation) or
+// wsum_a - 2 sum_{q in b} w_a[q] (Z representation).
+//
+// Evolution multiplies by exp(-i s E) formed as a product of factors: the
+// row factor exp(-i s rho_a) (with exp(-i s wsum_a) in Z form), the column
+// factor (precomputed per column) and one factor per occupied orbital of b,
+// exp(-i s w_a[q]) or exp(2 i s w_a[q]), tabled per row in LDS.  This costs a
+// few complex multiplies per element instead of an FP64 sin/cos.
+//
+// Each thread walks its columns two at a time with the next column's loads
+// in flight while the current one is formed and stored; loads read a clamped
+// column (always in bounds) and only stores are guarded, so that the memory
+// counters never wait on a branch.
 template <typename T, int R, int MODE>
 __global__ void __launch_bounds__(256)
     diag_kernel(const Cx<T>* src, Cx<T>* dst, long ld, int n_rows,
-                int n_cols, const uint64_t* __restrict__ row_str,
-                const uint64_t* __restrict__ col_str, const double* __restrict__ gamma,
+                int n_cols, int n_cols_blk, const uint64_t* __restrict__ row_str,
+                const uint64_t* __restrict__ col_str, const double* __restrict__ colv,
                 const DiagHeader* __restrict__ hdr, const double* __restrict__ coef) {
+  constexpr int NT = 256;
+  constexpr int LPR = NT / R;  // lanes per row in the row-term reduction (<= 64)
+  static_assert(LPR <= 64 && (LPR & (LPR - 1)) == 0, "R must be a power of two >= 4");
   __shared__ double w_row[R][64];
+  __shared__ Cx<double> fac[R][64];  // evolution: per-orbital factors
   __shared__ double rho[R];
-  __shared__ double wsum[R];
+  __shared__ Cx<double> rowf[R];     // evolution: row factors
   const int n = hdr->norb;
   const int z = hdr->z_rep;
   const int flags = hdr->flags;
   const double scale = hdr->scale;
   const int r0 = blockIdx.x * R;
+  const int c_lo = blockIdx.y * n_cols_blk;
+  const int c_hi = min(n_cols, c_lo + n_cols_blk);
   const double* A_row = coef;
   const double* X = coef + 2 * n * n;
   const double* lin_row = coef + 3 * n * n;
-  // per-row terms
-  for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
-    const int r = t / n, jp = t % n;
+  const int tid = threadIdx.x;
+
+  // cross vectors w_a[q] (rows beyond n_rows get zeros and are never stored)
+  for (int t = tid; t < R * n; t += NT) {
+    const int r = t / n, q = t % n;
     double acc = 0.0;
     if (r0 + r < n_rows && (flags & DF_CROSS)) {
       const uint64_t bits = row_str[r0 + r];
       for (int i = 0; i < n; ++i) {
         const int occ = (bits >> i) & 1;
         if (z)
-          acc += (occ ? -1.0 : 1.0) * X[i * n + jp];
+          acc += (occ ? -1.0 : 1.0) * X[i * n + q];
         else if (occ)
-          acc += X[i * n + jp];
+          acc += X[i * n + q];
       }
     }
-    w_row[r][jp] = acc;
+    w_row[r][q] = acc;
   }
-  if (threadIdx.x < R) {
-    const int r = threadIdx.x;
+  // row terms: LPR lanes per row, each summing a share of the (j, k >= j) pairs
+  {
+    const int r = tid / LPR, l = tid % LPR;
     double acc = 0.0;
     if (r0 + r < n_rows) {
       const uint64_t bits = row_str[r0 + r];
       if (flags & DF_QUAD) {
-        for (int j = 0; j < n; ++j) {
-          const int oj = (bits >> j) & 1;
-          for (int k = j; k < n; ++k) {
-            const int ok = (bits >> k) & 1;
-            if (z) {
-              if (k > j) acc += ((oj ^ ok) ? -1.0 : 1.0) * A_row[j * n + k];
-            } else if (oj && ok) {
-              acc += A_row[j * n + k];
-            }
+        for (int p = l; p < n * n; p += LPR) {
+          const int j = p / n, k = p % n;
+          if (k < j) continue;
+          const int oj = (bits >> j) & 1, ok = (bits >> k) & 1;
+          if (z) {
+            if (k > j) acc += ((oj ^ ok) ? -1.0 : 1.0) * A_row[j * n + k];
+          } else if (oj && ok) {
+            acc += A_row[j * n + k];
           }
         }
       }
-      if (flags & DF_LIN) {
-        for (int j = 0; j < n; ++j)
+      if (flags & DF_LIN)
+        for (int j = l; j < n; j += LPR)
           if ((bits >> j) & 1) acc += lin_row[j];
-      }
     }
-    rho[r] = acc;
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if (l == 0) rho[r] = acc;
   }
   __syncthreads();
-  if (threadIdx.x < R) {
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) s += w_row[threadIdx.x][j];
-    wsum[threadIdx.x] = s;
+  if (MODE == DIAG_EVOLVE) {
+    // per-orbital factors and row factors, angle = -scale * term
+    for (int t = tid; t < R * n; t += NT) {
+      const int r = t / n, q = t % n;
+      const double a = -scale * (z ? -2.0 * w_row[r][q] : w_row[r][q]);
+      double sn, cs;
+      sincos(a, &sn, &cs);
+      fac[r][q] = Cx<double>{cs, sn};
+    }
+    if (tid < R) {
+      double e = rho[tid];
+      if (z && (flags & DF_CROSS)) {
+        double ws = 0.0;
+        for (int q = 0; q < n; ++q) ws += w_row[tid][q];
+        e += ws;
+      }
+      double sn, cs;
+      sincos(-scale * e, &sn, &cs);
+      rowf[tid] = Cx<double>{cs, sn};
+    }
+  } else if (z && (flags & DF_CROSS) && tid < R) {
+    double ws = 0.0;
+    for (int q = 0; q < n; ++q) ws += w_row[tid][q];
+    rho[tid] += ws;
+    for (int q = 0; q < n; ++q) w_row[tid][q] *= -2.0;
   }
   __syncthreads();
 
-  for (int c = threadIdx.x; c < n_cols; c += blockDim.x) {
-    const uint64_t cb = col_str[c];
-    const double g = gamma ? gamma[c] : 0.0;
-    // all loads of this column first: src may alias dst (in-place evolution), so
-    // a load placed after the previous row's store could not be issued early
-    Cx<T> vin[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (r0 + r < n_rows) vin[r] = src[(long)(r0 + r) * ld + c];
+  struct Col {
+    Cx<T> v[R];
+    uint64_t bits;
+    double g0, g1;  // evolution: column factor (cos, sin); else gamma, unused
+  };
+  auto load = [&](int c, Col& k) {
+    const int cc = min(c, c_hi - 1);
+    k.bits = col_str[cc];
+    if (MODE == DIAG_EVOLVE) {
+      k.g0 = colv ? colv[2 * cc] : 1.0;
+      k.g1 = colv ? colv[2 * cc + 1] : 0.0;
+    } else {
+      k.g0 = colv ? colv[cc] : 0.0;
+      k.g1 = 0.0;
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      if (r0 + r >= n_rows) break;
-      double cross = 0.0;
-      if (flags & DF_CROSS) {
-        uint64_t bits = cb;
-        while (bits) {
-          const int jp = __builtin_ctzll(bits);
-          bits &= bits - 1;
-          cross += w_row[r][jp];
-        }
-        if (z) cross = wsum[r] - 2.0 * cross;
+      const int rr = min(r0 + r, n_rows - 1);
+      k.v[r] = src[(long)rr * ld + cc];
+    }
+  };
+  auto form = [&](int c, const Col& k) {
+    if (c >= c_hi) return;
+    if (MODE == DIAG_EVOLVE) {
+      double pr[R], pi[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const Cx<double> f = rowf[r];
+        pr[r] = f.re * k.g0 - f.im * k.g1;
+        pi[r] = f.re * k.g1 + f.im * k.g0;
       }
-      const double E = rho[r] + g + cross;
-      const long idx = (long)(r0 + r) * ld + c;
-      const Cx<T> v = vin[r];
-      Cx<T> o;
-      if (MODE == DIAG_EVOLVE) {
-        double sn, cs;
-        const double ang = -scale * E;
-        if (sizeof(T) == 8) {
-          sincos(ang, &sn, &cs);
-        } else {
-          const double red = ang - 6.283185307179586 * rint(ang * 0.15915494309189535);
-          float sf, cf;
-          sincosf((float)red, &sf, &cf);
-          sn = sf;
-          cs = cf;
+      if (flags & DF_CROSS) {
+        uint64_t bits = k.bits;
+        while (bits) {
+          const int q = __builtin_ctzll(bits);
+          bits &= bits - 1;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const Cx<double> f = fac[r][q];
+            const double t = pr[r] * f.re - pi[r] * f.im;
+            pi[r] = pr[r] * f.im + pi[r] * f.re;
+            pr[r] = t;
+          }
         }
-        o.re = v.re * (T)cs - v.im * (T)sn;
-        o.im = v.re * (T)sn + v.im * (T)cs;
-      } else {
-        const T f = (T)(scale * E);
-        o.re = f * v.re;
-        o.im = f * v.im;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r0 + r >= n_rows) break;
+        const Cx<T> v = k.v[r];
+        Cx<T> o;
+        o.re = v.re * (T)pr[r] - v.im * (T)pi[r];
+        o.im = v.re * (T)pi[r] + v.im * (T)pr[r];
+        dst[(long)(r0 + r) * ld + c] = o;
+      }
+    } else {
+      double cross[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) cross[r] = 0.0;
+      if (flags & DF_CROSS) {
+        uint64_t bits = k.bits;
+        while (bits) {
+          const int q = __builtin_ctzll(bits);
+          bits &= bits - 1;
+#pragma unroll
+          for (int r = 0; r < R; ++r) cross[r] += w_row[r][q];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r0 + r >= n_rows) break;
+        const long idx = (long)(r0 + r) * ld + c;
+        const T f = (T)(scale * (rho[r] + k.g0 + cross[r]));
+        const Cx<T> v = k.v[r];
+        Cx<T> o{f * v.re, f * v.im};
         if (MODE == DIAG_CONTRACT_ACC) {
           const Cx<T> p = dst[idx];
           o.re += p.re;
           o.im += p.im;
         }
+        dst[idx] = o;
       }
-      dst[idx] = o;
     }
+  };
+  // two columns per step: B's loads are in flight while A is formed, and A's
+  // next loads while B is formed
+  Col A, B;
+  int c = c_lo + tid;
+  if (c < c_hi) load(c, A);
+  for (; c < c_hi; c += 2 * NT) {
+    load(c + NT, B);
+    form(c, A);
+    load(c + 2 * NT, A);
+    form(c + NT, B);
   }
 }
 

@@ -18,6 +18,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -106,6 +107,43 @@ __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ co
   } else {
     gamma[c] = acc;
   }
+}
+
+// Factors of a diagonal Coulomb evolution for the strings of the state's row
+// sector (number representation), used by the fused store: for string a,
+// out[a][q] = exp(-i s sum_{p in a} X[p][q]) for q < norb and
+// out[a][norb] = exp(-i s rho_a) (quadratic and linear row terms).
+__global__ void diag_row_factor_kernel(int n_str, const uint64_t* __restrict__ str,
+                                       const DiagHeader* __restrict__ hdr,
+                                       const double* __restrict__ coef,
+                                       Cx<double>* __restrict__ out) {
+  const int n = hdr->norb, nf = n + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)n_str * nf) return;
+  const int a = (int)(t / nf), q = (int)(t % nf);
+  const int flags = hdr->flags;
+  const double* A_row = coef;
+  const double* X = coef + 2 * n * n;
+  const double* lin_row = coef + 3 * n * n;
+  const uint64_t bits = str[a];
+  double e = 0.0;
+  if (q < n) {
+    if (flags & DF_CROSS)
+      for (int p = 0; p < n; ++p)
+        if ((bits >> p) & 1) e += X[p * n + q];
+  } else {
+    if (flags & DF_QUAD)
+      for (int j = 0; j < n; ++j)
+        if ((bits >> j) & 1)
+          for (int k = j; k < n; ++k)
+            if ((bits >> k) & 1) e += A_row[j * n + k];
+    if (flags & DF_LIN)
+      for (int j = 0; j < n; ++j)
+        if ((bits >> j) & 1) e += lin_row[j];
+  }
+  double sn, cs;
+  sincos(-hdr->scale * e, &sn, &cs);
+  out[t] = Cx<double>{cs, sn};
 }
 
 namespace {
@@ -359,13 +397,24 @@ struct Program {
     if (c.ring.armed[slot]) HIP_CHECK(hipEventSynchronize(c.ring.ev[slot]));
     c.ring.armed[slot] = false;
   }
+  size_t scratch_lo = ProgramRing::kSlotBytes;  // device-only scratch grows down from the top
   // Appends n bytes (16-byte aligned), returns the offset.
   size_t append(const void* src, size_t n) {
     size_t off = (used + 15) & ~size_t(15);
-    require(off + n <= ProgramRing::kSlotBytes, FFSIM_ERR_VALUE, "gate program too large");
+    require(off + n <= scratch_lo, FFSIM_ERR_VALUE, "gate program too large");
     if (src) std::memcpy(ctx->ring.h[slot] + off, src, n);
     used = off + n;
     return off;
+  }
+  // Reserves n bytes of device-only scratch (written by kernels, never copied
+  // from the host); false when the slot has no room left.
+  bool reserve(size_t n, size_t* off) {
+    if (n > scratch_lo) return false;
+    const size_t o = (scratch_lo - n) & ~size_t(15);
+    if (o < ((used + 15) & ~size_t(15))) return false;
+    scratch_lo = o;
+    *off = o;
+    return true;
   }
   template <typename T>
   T* dev(size_t off) const {
@@ -447,29 +496,39 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
   return best;
 }
 
-// LDS of a phase pass beyond the tile: orbital phases and one string per row.
-size_t diag_lds_bytes(bool diag, int n_diag, int rows) {
-  if (!diag) return 0;
-  return 2 * kMaxDiagOrbitals * sizeof(double) + (size_t)rows * (n_diag <= 32 ? 4 : 8);
+// LDS of a phase pass beyond the tile: orbital phases and one string per row,
+// and with a fused diagonal the batch columns' factors.
+size_t diag_lds_bytes(bool diag, bool fuse, int n_diag, int rows, int W, int n_fac) {
+  if (!diag && !fuse) return 0;
+  size_t b = 2 * kMaxDiagOrbitals * sizeof(double) + align16((size_t)rows * (n_diag <= 32 ? 4 : 8));
+  if (fuse) b += (size_t)W * n_fac * 2 * sizeof(double);
+  return b;
 }
 
-template <typename T, int W, int MODE, int W2, int NTHR>
+// LDS of a fiber launch (the kernel's layout); n_waves sizes the wave schedule.
+size_t fiber_lds_bytes(size_t elem, int W, int n_waves, int rows, int gd_cap, int gm_cap,
+                       bool diag, bool fuse, int n_diag, int n_fac) {
+  const int S = (W >= 8) ? W + 1 : W;
+  return align16((size_t)rows * S * elem) + align16((size_t)rows * 4) +
+         kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
+         align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * n_waves * 8 +
+         align16((size_t)gd_cap * 4) + diag_lds_bytes(diag, fuse, n_diag, rows, W, n_fac);
+}
+
+template <typename T, int W, int MODE, int W2, int NTHR, bool FUSE = false>
 void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
                     const DevWindow& win, bool all_fibers, const DevGroups& gr, const LoSlots& los,
                     int gd_cap, int gm_cap, const SubPass* d_subs, int n_sub,
                     const LayerPair* d_layers, int n_layers, const double* d_diag,
-                    const uint64_t* d_strings, int n_diag, size_t lds_max, hipStream_t stream) {
-  constexpr int S = (W >= 8) ? W + 1 : W;
+                    const uint64_t* d_strings, int n_diag, size_t lds_max, hipStream_t stream,
+                    const DiagFuse& fz = DiagFuse{}) {
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
   if (n_f == 0 || n_batch == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
-  const int gc_cap = gd_cap;  // every chunk holds at least one group
-  const size_t lds = align16((size_t)rows * S * sizeof(Cx<T>)) + align16((size_t)rows * 4) +
-                     kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
-                     align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * (NTHR / 64) * 8 +
-                     align16((size_t)gc_cap * 4) + diag_lds_bytes(d_diag != nullptr, n_diag, rows);
+  const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, NTHR / 64, rows, gd_cap, gm_cap,
+                                     d_diag != nullptr, FUSE, n_diag, fz.n_fac);
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
-  auto kern = fiber_kernel<T, W, MODE, NTHR, W2>;
+  auto kern = fiber_kernel<T, W, MODE, NTHR, W2, FUSE>;
   const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
   static bool attr_set = false;
   if (!attr_set) {
@@ -493,7 +552,7 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
                      ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(sched), los,
                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap,
-                     gm_cap);
+                     gm_cap, fz);
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
 }
@@ -504,7 +563,7 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                   const LoSlots& los, int gd_cap, int gm_cap,
                   const SubPass* d_subs, int n_sub, const LayerPair* d_layers, int n_layers,
                   const double* d_diag, const uint64_t* d_strings, int n_diag, size_t lds_max,
-                  int W, hipStream_t stream) {
+                  int W, hipStream_t stream, const DiagFuse* fz) {
   // gr.w2 < kMaxSubWindow only for windows narrower than the sub-window
 #define FFSIM_FIBER_LAUNCH_N(WW, MM, W2, NT)                                                    \
   launch_fiber_t<T, WW, MM, W2, NT>(src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers,  \
@@ -512,12 +571,27 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                                     d_diag, d_strings, n_diag, lds_max, stream)
   // 512 threads when one block fills the CU's LDS, 256 when two or more fit
   const int rows_l = all_fibers ? win.max_rows : win.max_rows_active;
-  const int S_l = (W >= 8) ? W + 1 : W;
-  const size_t lds_l = align16((size_t)rows_l * S_l * sizeof(Cx<T>)) + align16((size_t)rows_l * 4) +
-                       kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
-                       align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * 16 * 8 +
-                       align16((size_t)gd_cap * 4) + diag_lds_bytes(d_diag != nullptr, n_diag, rows_l);
+  const size_t lds_l = fiber_lds_bytes(sizeof(Cx<T>), W, 16, rows_l, gd_cap, gm_cap,
+                                       d_diag != nullptr, fz != nullptr, n_diag,
+                                       fz ? fz->n_fac : 0);
   const bool wide = 2 * lds_l > lds_max;
+  if (fz) {
+    // the fused diagonal store exists for the c128 production shape only
+    if constexpr (std::is_same<T, double>::value) {
+      require(W == 8 && mode == MODE_ROWS && gr.w2 == kMaxSubWindow && tstore, FFSIM_ERR_VALUE,
+              "fused diagonal: unsupported pass");
+      if (wide)
+        launch_fiber_t<double, 8, MODE_ROWS, kMaxSubWindow, 512, true>(
+            src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
+            d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
+      else
+        launch_fiber_t<double, 8, MODE_ROWS, kMaxSubWindow, 256, true>(
+            src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
+            d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
+      return;
+    }
+    throw Failure{FFSIM_ERR_VALUE, "fused diagonal: complex128 only"};
+  }
 #define FFSIM_FIBER_LAUNCH(WW, MM, W2)               \
   do {                                               \
     if (wide)                                        \
@@ -569,16 +643,16 @@ void fiber_pass(int dtype, int mode, const void* src, void* dst, long ld_src, lo
                 const DevGroups& gr, const LoSlots& los, int gd_cap, int gm_cap,
                 const SubPass* d_subs, int n_sub, const LayerPair* d_layers,
                 int n_layers, const double* d_diag, const uint64_t* d_strings, int n_diag,
-                Context& ctx, hipStream_t stream) {
+                Context& ctx, hipStream_t stream, const DiagFuse* fz = nullptr) {
   const int W = tile_width(dtype);
   if (dtype == FFSIM_C128)
     launch_fiber<double>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
                          los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, ctx.lds_max, W,
-                         stream);
+                         stream, fz);
   else
     launch_fiber<float>(mode, src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr,
                         los, gd_cap, gm_cap, d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, ctx.lds_max, W,
-                        stream);
+                        stream, fz);
 }
 
 // One spin sector's orbital rotation, prepared on the host.
@@ -602,9 +676,49 @@ SectorRotation prepare_rotation(const cdouble* mat, int norb) {
 // strings of this sector on rows (MODE_ROWS, ld = n_batch) or on columns
 // (MODE_COLS).  If tstore_dst is non-null the final pass writes the
 // transposed state there (MODE_ROWS only).
+// Diagonal operator helper.  rows/cols strings are for the state's row and
+// column sectors; matrices are given for (row sector, col sector, cross).
+struct DiagSpec {
+  int norb = 0, z_rep = 0, flags = 0;
+  double scale = 1.0;
+  std::vector<double> A_row, A_col, X, lin_row, lin_col;
+};
+
+// Appends a diagonal operator's header and coefficients to a call's program;
+// returns their offsets.
+std::pair<size_t, size_t> append_diag_params(Program& prog, const DiagSpec& spec) {
+  DiagHeader h{};
+  h.norb = spec.norb;
+  h.z_rep = spec.z_rep;
+  h.flags = spec.flags;
+  h.scale = spec.scale;
+  const size_t hoff = prog.append(&h, sizeof(h));
+  const size_t nn = (size_t)spec.norb * spec.norb;
+  std::vector<double> coef(3 * nn + 2 * spec.norb, 0.0);
+  auto put = [&](const std::vector<double>& v, size_t off, size_t n) {
+    if (!v.empty()) std::copy(v.begin(), v.begin() + n, coef.begin() + off);
+  };
+  put(spec.A_row, 0, nn);
+  put(spec.A_col, nn, nn);
+  put(spec.X, 2 * nn, nn);
+  put(spec.lin_row, 3 * nn, spec.norb);
+  put(spec.lin_col, 3 * nn + spec.norb, spec.norb);
+  const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
+  return {hoff, coff};
+}
+
+// Factor tables of a diagonal Coulomb evolution to fuse into the final
+// transposing pass (offsets in the call's program; see DiagFuse).
+struct FuseTables {
+  size_t bfac_off = 0, rfac_off = 0;
+  int n_fac = 0;
+};
+
 void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, long ld,
                 int n_batch, int norb, int nocc, const SectorRotation& rot, void* tstore_dst,
-                long ld_t, hipStream_t stream, std::vector<std::function<void()>>& launches) {
+                long ld_t, hipStream_t stream, std::vector<std::function<void()>>& launches,
+                const FuseTables* fuse = nullptr, bool* fused = nullptr) {
+  if (fused) *fused = false;
   const int w = (mode == MODE_COLS) ? norb : choose_window(ctx, norb, nocc, dtype);
   const bool tstore = tstore_dst != nullptr;
   std::vector<PlannedPass> passes =
@@ -655,20 +769,40 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     void* dst = ts ? tstore_dst : state;
     const long ld_dst = ts ? ld_t : ld;
     const bool all = ts || dg;
+    // fuse the diagonal into this pass when the kernel's fused layout applies
+    bool fz_on = false;
+    if (ts && fuse && dtype == FFSIM_C128 && mode == MODE_ROWS && tile_width(dtype) == 8 &&
+        gr.w2 == kMaxSubWindow && n_sub > 0) {
+      const int rows = win.max_rows;
+      const size_t lds = fiber_lds_bytes(16, 8, 16, rows, gd_cap, gm_cap, dg, true, norb,
+                                         fuse->n_fac);
+      const int nthr = 2 * lds > ctx.lds_max ? 512 : 256;
+      fz_on = lds <= ctx.lds_max && rows <= 2 * nthr && 8 * fuse->n_fac <= nthr;
+    }
+    if (fz_on && fused) *fused = true;
+    const FuseTables ft = fz_on ? *fuse : FuseTables{};
     launches.push_back([=, &ctx, &prog, &win, &gr, &ds]() {
+      DiagFuse dfz{};
+      if (fz_on)
+        dfz = DiagFuse{prog.dev<Cx<double>>(ft.bfac_off), prog.dev<Cx<double>>(ft.rfac_off),
+                       ft.n_fac};
       fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, gr, los, gd_cap,
                  gm_cap, prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff), n_layers,
-                 dg ? prog.dev<double>(diag_off) : nullptr, ds.d, norb, ctx, stream);
+                 dg ? prog.dev<double>(diag_off) : nullptr, ds.d, norb, ctx, stream,
+                 fz_on ? &dfz : nullptr);
     });
   }
 }
 
 // Full spinful orbital rotation (both sectors) on a dim_a x dim_b state.
-void orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int norb, int na,
+// With `then_diag`, the diagonal Coulomb evolution that follows the rotation is
+// fused into the final pass when that pass is a transposing c128 row pass and
+// everything fits; returns true when it was (the caller applies it otherwise).
+bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int norb, int na,
                            int nb, const cdouble* mat_a, const cdouble* mat_b,
-                           hipStream_t stream) {
+                           hipStream_t stream, const DiagSpec* then_diag = nullptr) {
   const long dim_a = (long)binom(norb, na), dim_b = (long)binom(norb, nb);
-  if (dim_a == 0 || dim_b == 0 || norb == 0) return;
+  if (dim_a == 0 || dim_b == 0 || norb == 0) return false;
   require(dim_a < (1l << 31) && dim_b < (1l << 31), FFSIM_ERR_VALUE, "dimension too large");
   SectorRotation ra, rb;
   if (mat_a) ra = prepare_rotation(mat_a, norb);
@@ -683,6 +817,36 @@ void orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
   std::vector<std::function<void()>> launches;
   const bool beta_transposed = mat_b && !beta_cols;
   if (beta_transposed) require(work != nullptr, FFSIM_ERR_VALUE, "work buffer required");
+  // factor tables of the fused diagonal: alpha strings are the final pass's
+  // batch columns, beta strings its tile rows
+  FuseTables ft;
+  bool want_fuse = then_diag && then_diag->flags && !then_diag->z_rep &&
+                   then_diag->norb == norb && dtype == FFSIM_C128 && beta_transposed &&
+                   norb + 1 <= 64;
+  if (want_fuse) {
+    ft.n_fac = norb + 1;
+    want_fuse = prog.reserve((size_t)dim_a * ft.n_fac * sizeof(Cx<double>), &ft.bfac_off) &&
+                prog.reserve((size_t)dim_b * sizeof(Cx<double>), &ft.rfac_off);
+  }
+  if (want_fuse) {
+    const std::pair<size_t, size_t> par = append_diag_params(prog, *then_diag);
+    const size_t hoff = par.first, coff = par.second;
+    const DevStrings& sa = dev_strings(ctx, norb, na);
+    const DevStrings& sb = dev_strings(ctx, norb, nb);
+    const int nf = ft.n_fac;
+    launches.push_back([=, &prog, &sa, &sb]() {
+      const long n_a = dim_a * nf;
+      hipLaunchKernelGGL(diag_row_factor_kernel, dim3((unsigned)((n_a + 255) / 256)), dim3(256),
+                         0, stream, (int)dim_a, sa.d, prog.dev<DiagHeader>(hoff),
+                         prog.dev<double>(coff), prog.dev<Cx<double>>(ft.bfac_off));
+      HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((dim_b + 255) / 256)), dim3(256),
+                         0, stream, (int)dim_b, sb.d, prog.dev<DiagHeader>(hoff),
+                         prog.dev<double>(coff), prog.dev<double>(ft.rfac_off), 1);
+      HIP_CHECK(hipGetLastError());
+    });
+  }
+  bool fused = false;
   if (mat_a || beta_transposed) {
     SectorRotation empty;
     const SectorRotation& r = mat_a ? ra : empty;
@@ -695,21 +859,15 @@ void orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
                  stream, launches);
     } else {
       run_sector(ctx, prog, dtype, MODE_ROWS, work, dim_a, (int)dim_a, norb, nb, rb, vec, dim_b,
-                 stream, launches);
+                 stream, launches, want_fuse ? &ft : nullptr, &fused);
     }
   }
   prog.commit();
   for (auto& f : launches) f();
   prog.finish();
+  return fused;
 }
 
-// Diagonal operator helper.  rows/cols strings are for the state's row and
-// column sectors; matrices are given for (row sector, col sector, cross).
-struct DiagSpec {
-  int norb = 0, z_rep = 0, flags = 0;
-  double scale = 1.0;
-  std::vector<double> A_row, A_col, X, lin_row, lin_col;
-};
 
 void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int norb, int nrow_occ,
                int ncol_occ, const DiagSpec& spec, hipStream_t stream) {
@@ -719,23 +877,8 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   const DevStrings& rs = dev_strings(ctx, norb, nrow_occ);
   const DevStrings& cs = dev_strings(ctx, norb, ncol_occ);
   Program prog(ctx, stream);
-  DiagHeader h{};
-  h.norb = norb;
-  h.z_rep = spec.z_rep;
-  h.flags = spec.flags;
-  h.scale = spec.scale;
-  const size_t hoff = prog.append(&h, sizeof(h));
-  const size_t nn = (size_t)norb * norb;
-  std::vector<double> coef(3 * nn + 2 * norb, 0.0);
-  auto put = [&](const std::vector<double>& v, size_t off, size_t n) {
-    if (!v.empty()) std::copy(v.begin(), v.begin() + n, coef.begin() + off);
-  };
-  put(spec.A_row, 0, nn);
-  put(spec.A_col, nn, nn);
-  put(spec.X, 2 * nn, nn);
-  put(spec.lin_row, 3 * nn, norb);
-  put(spec.lin_col, 3 * nn + norb, norb);
-  const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
+  require(spec.norb == norb, FFSIM_ERR_VALUE, "diagonal operator of another norb");
+  const auto [hoff, coff] = append_diag_params(prog, spec);
   // per column: gamma (contractions) or its phase factor (cos, sin) (evolution)
   const int evolve = op == DIAG_EVOLVE ? 1 : 0;
   const size_t goff = prog.append(nullptr, (size_t)n_cols * (evolve ? 2 : 1) * sizeof(double));
@@ -816,10 +959,13 @@ void diag_coulomb_sequence_impl(Context& ctx, void* vec, void* work, int dtype, 
                                 const double* const* mats /* aa, ab, bb per term (3 ptrs) */,
                                 const double* times, int z_rep, hipStream_t st) {
   for (int t = 0; t < n_terms; ++t) {
-    if (rots[t]) orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[t], rots[t], st);
     DiagSpec s = diag_coulomb_spec(norb, mats[3 * t], mats[3 * t + 1], mats[3 * t + 2], z_rep,
                                    times[t]);
-    if (s.flags) diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, norb, na, nb, s, st);
+    bool fused = false;
+    if (rots[t])
+      fused = orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[t], rots[t], st,
+                                    s.flags ? &s : nullptr);
+    if (s.flags && !fused) diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, norb, na, nb, s, st);
   }
   if (rots[n_terms])
     orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[n_terms], rots[n_terms], st);

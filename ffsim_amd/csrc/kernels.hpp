@@ -231,6 +231,17 @@ struct GroupSlot {
 
 __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// A diagonal Coulomb evolution fused into the store of a transposing pass
+// (the last pass of the second sector's rotation).  Element (tile string i,
+// batch string j) is multiplied by exp(-i t E) formed from complex FP64
+// factors: rfac of string i, the batch term bfac[j][norb], and bfac[j][q] for
+// every occupied orbital q of string i.
+struct DiagFuse {
+  const Cx<double>* bfac;  // [n_batch][n_fac]: per orbital q < norb, then the batch term
+  const Cx<double>* rfac;  // [strings of the tile sector]
+  int n_fac;               // norb + 1 (at most 64)
+};
+
 // Element (string i, batch j) of a fiber tile lives at
 //   MODE_ROWS: state[row_table[i] * ld + b0 + j]   (strings index rows)
 //   MODE_COLS: state[(b0 + j) * ld + row_table[i]] (strings index columns)
@@ -243,7 +254,7 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size
 // LDS, the next tile's loads are already in flight into registers.
 //
 // LDS: tile[max_rows * S] | rows[max_rows] | slots | desc | members | wave schedule
-template <typename T, int W, int MODE, int NTHR, int W2>
+template <typename T, int W, int MODE, int NTHR, int W2, bool FUSE = false>
 __global__ void __launch_bounds__(NTHR)
     fiber_kernel(const Cx<T>* src, Cx<T>* dst, long ld_src, long ld_dst, int n_batch, int tstore,
                  const Fiber* __restrict__ fibers, int n_fibers,
@@ -251,7 +262,8 @@ __global__ void __launch_bounds__(NTHR)
                  const SubPass* __restrict__ subs_g, int n_sub,
                  const LayerPair* __restrict__ layers_g, int n_layers,
                  const double* __restrict__ diag, const uint64_t* __restrict__ strings,
-                 int n_diag, int max_rows, int gd_cap, int gm_cap) {
+                 int n_diag, int max_rows, int gd_cap, int gm_cap, DiagFuse fz) {
+  static_assert(!FUSE || MODE == MODE_ROWS, "the fused diagonal store is a row-mode store");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int S = (W >= 8) ? W + 1 : W;  // LDS stride of one string's batch
   constexpr int RSTEP = NTHR / W;          // rows per sweep in MODE_ROWS
@@ -282,6 +294,11 @@ __global__ void __launch_bounds__(NTHR)
   uint32_t* rstr32 = reinterpret_cast<uint32_t*>(smem_raw + off);
   uint64_t* rstr64 = reinterpret_cast<uint64_t*>(smem_raw + off);
   const bool narrow_str = n_diag <= 32;
+  // fused diagonal: the batch strings' factors of the current tile, with the
+  // orbital phases of the rotation folded into the per-orbital ones
+  Cx<double>* ffac = reinterpret_cast<Cx<double>*>(
+      smem_raw + off + align16((size_t)max_rows * (narrow_str ? 4 : 8)));
+  const bool have_str = diag != nullptr || FUSE;
 
   const int tid = threadIdx.x;
   const int n_bt = (n_batch + W - 1) / W;
@@ -291,9 +308,19 @@ __global__ void __launch_bounds__(NTHR)
   // stage the row addresses and group tables of fiber f
   // The group tables and wave schedule depend only on the fiber's electron count
   // in the window (kp); `tables` is false when the previous fiber had the same kp.
+  // fused diagonal: the tile strings' factors of this thread's store rows
+  // (i = tid, tid + NTHR), fixed while the fiber is
+  Cx<double> rf[2] = {{1.0, 0.0}, {1.0, 0.0}};
   auto stage_fiber = [&](const Fiber& fd, bool tables) {
     for (int i = tid; i < fd.n_rows; i += NTHR) rows[i] = row_table[fd.row_off + i];
-    if (diag) {
+    if (FUSE) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int i = tid + k * NTHR;
+        rf[k] = i < fd.n_rows ? fz.rfac[row_table[fd.row_off + i]] : Cx<double>{1.0, 0.0};
+      }
+    }
+    if (have_str) {
       for (int i = tid; i < fd.n_rows; i += NTHR) {
         const uint64_t sv = strings[rows[i]];
         if (narrow_str)
@@ -429,6 +456,42 @@ __global__ void __launch_bounds__(NTHR)
           dst[(long)rows[i] * ld_dst + b0 + j] = v;
         }
       }
+    } else if (FUSE) {
+      // orbital phases and the diagonal evolution: one product per row and batch
+      // column over the row string's occupied orbitals (8 independent chains)
+      for (int i = tid; i < nr; i += NTHR) {
+        const Cx<double> r0f = i < NTHR ? rf[0] : rf[1];
+        double Pr[W], Pi[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const Cx<double> b = ffac[j * fz.n_fac + fz.n_fac - 1];
+          Pr[j] = r0f.re * b.re - r0f.im * b.im;
+          Pi[j] = r0f.re * b.im + r0f.im * b.re;
+        }
+        uint64_t bits = narrow_str ? (uint64_t)rstr32[i] : rstr64[i];
+        while (bits) {
+          const int q = __builtin_ctzll(bits);
+          bits &= bits - 1;
+#pragma unroll
+          for (int j = 0; j < W; ++j) {
+            const Cx<double> f = ffac[j * fz.n_fac + q];
+            const double t = Pr[j] * f.re - Pi[j] * f.im;
+            Pi[j] = Pr[j] * f.im + Pi[j] * f.re;
+            Pr[j] = t;
+          }
+        }
+        const long r = rows[i];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          if (j < nb) {
+            Cx<T> v = tile[i * S + j];
+            const T t = v.re * (T)Pr[j] - v.im * (T)Pi[j];
+            v.im = v.re * (T)Pi[j] + v.im * (T)Pr[j];
+            v.re = t;
+            dst[(long)(b0 + j) * ld_dst + r] = v;
+          }
+        }
+      }
     } else {
       for (int i = tid; i < nr; i += NTHR) {
         T pr = T(1), pim = T(0);
@@ -450,6 +513,25 @@ __global__ void __launch_bounds__(NTHR)
     }
   };
 
+  // fused diagonal: thread t < W * n_fac holds factor q = t % n_fac of batch
+  // column j = t / n_fac (read from a clamped, always valid address)
+  auto load_bfac = [&](int b0x) -> Cx<double> {
+    if (!FUSE) return Cx<double>{1.0, 0.0};
+    const int t = min(tid, W * fz.n_fac - 1);
+    const int j = t / fz.n_fac, q = t % fz.n_fac;
+    return fz.bfac[(long)min(b0x + j, n_batch - 1) * fz.n_fac + q];
+  };
+  auto bfac_to_lds = [&](Cx<double> v, int b0x) {
+    if (!FUSE || tid >= W * fz.n_fac) return;
+    const int j = tid / fz.n_fac, q = tid % fz.n_fac;
+    if (b0x + j >= n_batch) v = Cx<double>{1.0, 0.0};
+    if (diag && q < fz.n_fac - 1) {
+      const double dr = dph[2 * q], di = dph[2 * q + 1];
+      v = Cx<double>{v.re * dr - v.im * di, v.re * di + v.im * dr};
+    }
+    ffac[tid] = v;
+  };
+
   // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so block
   // b runs on XCD b % 8.  Renumbering makes each XCD's blocks take consecutive
   // tiles, i.e. neighbouring batch tiles of one fiber, whose rows share the
@@ -467,10 +549,12 @@ __global__ void __launch_bounds__(NTHR)
   stage_fiber(fd, true);
   __syncthreads();
   Cx<T> buf[CH];
+  Cx<double> bfn;
   {
     const int b0 = (int)(t % n_bt) * W;
     const int nb = min(W, n_batch - b0);
     load_rows(0, fd.n_rows, b0, nb);
+    if (FUSE) bfac_to_lds(load_bfac(b0), b0);
   }
   while (true) {
     const int nr = fd.n_rows;
@@ -494,6 +578,7 @@ __global__ void __launch_bounds__(NTHR)
       load_regs(buf, nrn, b0n, nbn, rows);
     else
       load_regs(buf, nrn, b0n, nbn, row_table + fdn.row_off);
+    if (FUSE) bfn = load_bfac(b0n);
 
     // ---- register sub-passes on the current tile
     const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
@@ -531,6 +616,7 @@ __global__ void __launch_bounds__(NTHR)
       __syncthreads();
     }
     regs_to_lds(buf, nrn);
+    if (FUSE) bfac_to_lds(bfn, b0n);
     load_rows(kCovered, nrn, b0n, nbn);
     t = tn;
   }

@@ -217,7 +217,8 @@ def test_contractions_and_linops(norb, nelec, rng):
 
 @pytest.mark.parametrize("norb,nelec,n_reps,final",
                          [(4, (2, 2), 1, False), (6, (3, 2), 2, True), (8, (4, 4), 3, True),
-                          (12, (6, 6), 4, False), (14, (7, 7), 2, True)])
+                          (12, (6, 6), 4, False), (14, (7, 7), 2, True), (13, (6, 5), 2, False),
+                          (13, (4, 7), 1, True)])
 def test_ucj_spin_balanced(norb, nelec, n_reps, final, rng):
     import ffsim_amd
 
@@ -301,12 +302,15 @@ def test_complex64_error_vs_fp64_oracle(norb, nelec, rng):
     assert infidelity < 1e-6
 
 
-@pytest.mark.parametrize("order,n_steps,z_rep", [(0, 2, False), (1, 2, False), (2, 1, True)])
-def test_trotter_double_factorized(order, n_steps, z_rep, rng):
+@pytest.mark.parametrize("norb,nelec,order,n_steps,z_rep",
+                         [(8, (4, 3), 0, 2, False), (8, (4, 3), 1, 2, False), (8, (4, 3), 2, 1, True),
+                          (13, (6, 5), 1, 1, False), (13, (6, 5), 0, 1, True)])
+def test_trotter_double_factorized(norb, nelec, order, n_steps, z_rep, rng):
+    """norb=13 runs the beta sector transposed, where (number representation)
+    the diagonal evolution is fused into the last pass of each rotation."""
     import ffsim_amd
 
-    norb, nelec = 8, (4, 3)
-    _case("trotter", order, n_steps, z_rep)
+    _case("trotter", norb, nelec, order, n_steps, z_rep)
     df = ffsim_amd.random.random_double_factorized_hamiltonian(norb, rank=5,
                                                                z_representation=z_rep, seed=rng)
     vec = _rand(norb, nelec, rng)

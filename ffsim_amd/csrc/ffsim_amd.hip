@@ -187,7 +187,8 @@ struct DevGroups {
 // event recorded behind the call's last kernel has completed.
 struct ProgramRing {
   static constexpr int kSlots = 16;
-  static constexpr size_t kSlotBytes = 8u << 20;
+  static constexpr size_t kSlotBytes = 8u << 20;      // host-written parameters (pinned)
+  static constexpr size_t kDevSlotBytes = 32u << 20;  // device slot: parameters + scratch
   char* h[kSlots] = {};
   char* d[kSlots] = {};
   hipEvent_t ev[kSlots] = {};
@@ -265,7 +266,7 @@ Context& context() {
   g_n_cu = std::max(1, prop.multiProcessorCount);
   for (int k = 0; k < ProgramRing::kSlots; ++k) {
     HIP_CHECK(hipHostMalloc((void**)&ctx->ring.h[k], ProgramRing::kSlotBytes, hipHostMallocDefault));
-    HIP_CHECK(hipMalloc((void**)&ctx->ring.d[k], ProgramRing::kSlotBytes));
+    HIP_CHECK(hipMalloc((void**)&ctx->ring.d[k], ProgramRing::kDevSlotBytes));
     HIP_CHECK(hipEventCreateWithFlags(&ctx->ring.ev[k], hipEventDisableTiming));
   }
   Context& ref = *ctx;
@@ -397,11 +398,12 @@ struct Program {
     if (c.ring.armed[slot]) HIP_CHECK(hipEventSynchronize(c.ring.ev[slot]));
     c.ring.armed[slot] = false;
   }
-  size_t scratch_lo = ProgramRing::kSlotBytes;  // device-only scratch grows down from the top
+  size_t scratch_lo = ProgramRing::kDevSlotBytes;  // device-only scratch grows down from the top
   // Appends n bytes (16-byte aligned), returns the offset.
   size_t append(const void* src, size_t n) {
     size_t off = (used + 15) & ~size_t(15);
-    require(off + n <= scratch_lo, FFSIM_ERR_VALUE, "gate program too large");
+    require(off + n <= std::min(scratch_lo, ProgramRing::kSlotBytes), FFSIM_ERR_VALUE,
+            "gate program too large");
     if (src) std::memcpy(ctx->ring.h[slot] + off, src, n);
     used = off + n;
     return off;

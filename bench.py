@@ -123,6 +123,28 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int,
     }
 
 
+def c64_error(op, state, norb: int, nelec) -> dict:
+    """One LUCJ application in complex64 and in complex128 from the same state
+    (the benchmark's state, normalized); max abs difference and
+    1 - |<a|b>|^2 / (<a|a><b|b>), computed on the device in FP64."""
+    import torch
+
+    a = state.to(torch.complex128, copy=True)
+    a /= torch.linalg.vector_norm(a)
+    b = a.to(torch.complex64)
+    op.apply_in_place(a, norb, nelec)
+    op.apply_in_place(b, norb, nelec)
+    b = b.to(torch.complex128)
+    max_abs = float((a - b).abs().max().item())
+    ov = torch.vdot(a, b).item()
+    na, nb = torch.vdot(a, a).real.item(), torch.vdot(b, b).real.item()
+    del a, b
+    torch.cuda.empty_cache()
+    return {"workload": "one LUCJ n_reps=4 application from the benchmark state",
+            "max_abs": max_abs, "infidelity": float(1.0 - abs(ov) ** 2 / (na * nb)),
+            "c64_norm_error": float(abs(math.sqrt(nb) - 1.0))}
+
+
 def copy_bandwidth(src, dst, reps: int = 10) -> float:
     """Device-to-device copy rate in GB/s (bytes read + bytes written), the
     practical HBM ceiling on this GPU for a streaming read-write kernel."""
@@ -533,6 +555,8 @@ def main() -> int:
             "achieved_GBs": round(diag_bytes / max(diag_ms, 1e-9) / 1e6, 1),
         },
     }
+    result["c64_vs_c128"] = c64_error(op, state, norb, nelec)
+    log(f"c64 vs c128 (one application): {result['c64_vs_c128']}")
     vec_host = None if args.no_cpu_baseline else state.to(torch.complex128).cpu().numpy()
     if not args.no_trotter:
         del state

@@ -507,7 +507,9 @@ size_t diag_lds_bytes(bool diag, bool fuse, int n_diag, int rows, int W, int n_f
   return b;
 }
 
-// LDS of a fiber launch (the kernel's layout); n_waves sizes the wave schedule.
+// LDS of a fiber launch (the kernel's layout; the tile stride is taken padded,
+// as in column mode, so row mode may reserve a little more than it uses);
+// n_waves sizes the wave schedule.
 size_t fiber_lds_bytes(size_t elem, int W, int n_waves, int rows, int gd_cap, int gm_cap,
                        bool diag, bool fuse, int n_diag, int n_fac) {
   const int S = (W >= 8) ? W + 1 : W;

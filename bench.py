@@ -428,6 +428,8 @@ def main() -> int:
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-c64-check", action="store_true",
+                    help="skip the c64-vs-c128 comparison (keeps profiler runs to one dtype)")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=400)
@@ -555,8 +557,9 @@ def main() -> int:
             "achieved_GBs": round(diag_bytes / max(diag_ms, 1e-9) / 1e6, 1),
         },
     }
-    result["c64_vs_c128"] = c64_error(op, state, norb, nelec)
-    log(f"c64 vs c128 (one application): {result['c64_vs_c128']}")
+    if not args.no_c64_check:
+        result["c64_vs_c128"] = c64_error(op, state, norb, nelec)
+        log(f"c64 vs c128 (one application): {result['c64_vs_c128']}")
     vec_host = None if args.no_cpu_baseline else state.to(torch.complex128).cpu().numpy()
     if not args.no_trotter:
         del state

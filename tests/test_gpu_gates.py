@@ -367,3 +367,39 @@ def test_norb18_rotation_round_trip_and_trotter_norm():
     assert abs(torch.linalg.vector_norm(psi).item() - 1.0) < 1e-12
     del psi
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 1e-8)])
+def test_lucj_norb16_round_trip(dtype, tol):
+    """Full-size LUCJ n_reps=4 at norb=16 (8,8) (12870^2, 2.65 GB in complex128):
+    the operator followed by its inverse (reversed layers, negated diagonal
+    Coulomb matrices) returns the state.  In complex128 this runs the fused
+    diagonal store of every rotation at the benchmark's shape."""
+    torch = _torch()
+    import ffsim_amd
+
+    norb, nelec = 16, (8, 8)
+    _case("lucj norb=16 round trip", dtype)
+    pairs = ([(p, p + 1) for p in range(norb - 1)], [(p, p) for p in range(norb)])
+    op = ffsim_amd.random.random_ucj_op_spin_balanced(norb, n_reps=4, interaction_pairs=pairs,
+                                                      seed=16)
+    inv = ffsim_amd.UCJOpSpinBalanced(
+        diag_coulomb_mats=-op.diag_coulomb_mats[::-1].copy(),
+        orbital_rotations=op.orbital_rotations[::-1].copy(),
+    )
+    dim = math.comb(norb, 8) ** 2
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(16)
+    cdt = torch.complex128 if dtype == "c128" else torch.complex64
+    psi = torch.randn(dim, dtype=torch.complex128, device="cuda", generator=gen)
+    psi /= torch.linalg.vector_norm(psi)
+    psi = psi.to(cdt)
+    ref = psi.clone()
+    op.apply_in_place(psi, norb, nelec)
+    assert (psi - ref).abs().max().item() > 1e-6  # it did act
+    inv.apply_in_place(psi, norb, nelec)
+    err = (psi - ref).abs().max().item()
+    print(f"lucj norb=16 {dtype} round trip |d|inf={err:.3e}", flush=True)
+    assert err < tol
+    del psi, ref
+    torch.cuda.empty_cache()

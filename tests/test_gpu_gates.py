@@ -403,3 +403,29 @@ def test_lucj_norb16_round_trip(dtype, tol):
     assert err < tol
     del psi, ref
     torch.cuda.empty_cache()
+
+
+def test_ucj_fused_diagonal_with_trivial_orbital_phases(rng):
+    """Orbital rotations that are single real Givens rotations decompose with
+    D exactly 1, so the fused diagonal store runs without orbital phases
+    (norb=13: the beta sector is transposed)."""
+    import ffsim_amd
+
+    norb, nelec = 13, (6, 5)
+    _case("ucj fused, trivial D", norb, nelec)
+    rots = []
+    for a, theta in ((2, 0.4), (9, -0.7)):
+        U = np.eye(norb, dtype=complex)
+        c, s = np.cos(theta), np.sin(theta)
+        U[a, a], U[a, a + 1], U[a + 1, a], U[a + 1, a + 1] = c, s, -s, c
+        _, d = ffsim_amd.linalg.givens_decomposition(U)
+        assert np.all(d == 1)
+        rots.append(U)
+    ref = ffsim_amd.random.random_ucj_op_spin_balanced(norb, n_reps=2, seed=rng)
+    op = ffsim_amd.UCJOpSpinBalanced(diag_coulomb_mats=ref.diag_coulomb_mats,
+                                     orbital_rotations=np.stack(rots))
+    vec = _rand(norb, nelec, rng)
+    expected = O.apply_ucj_spin_balanced(vec, op.diag_coulomb_mats, op.orbital_rotations, None,
+                                         norb, nelec)
+    got = ffsim_amd.apply_unitary(vec, op, norb, nelec)
+    np.testing.assert_allclose(got, expected, atol=TOL, rtol=0)

@@ -204,7 +204,7 @@ struct Context {
   std::map<std::pair<int, int>, DevStrings> strings;
   std::map<std::pair<int, int>, DevLinks> links;
   std::map<std::tuple<int, int, int, int>, DevWindow> windows;
-  std::map<int, DevGroups> groups;
+  std::map<std::pair<int, bool>, DevGroups> groups;  // (window width, merged complements)
   ProgramRing ring;
 };
 
@@ -353,11 +353,15 @@ const DevWindow& dev_window(Context& ctx, int norb, int nocc, int lo, int w) {
   return ctx.windows[key] = dw;
 }
 
-const DevGroups& dev_groups(Context& ctx, int w) {
+// Group tables of a window width.  c64 tables merge complemented groups into
+// the mirrored class (fewer, fuller chunks: 4.6% faster); the c128 kernel keeps
+// Re(sigma) in SGPRs, which complemented lanes cannot share (4% slower merged).
+const DevGroups& dev_groups(Context& ctx, int w, bool merge) {
   std::lock_guard<std::mutex> lk(ctx.mu);
-  auto it = ctx.groups.find(w);
+  const auto key = std::make_pair(w, merge);
+  auto it = ctx.groups.find(key);
   if (it != ctx.groups.end()) return it->second;
-  GroupTables T = make_group_tables(w, kMaxSubWindow);
+  GroupTables T = make_group_tables(w, kMaxSubWindow, merge);
   DevGroups dg;
   dg.desc = upload_table(T.desc);
   dg.members = upload_table(T.members);
@@ -367,7 +371,7 @@ const DevGroups& dev_groups(Context& ctx, int w) {
   dg.w = w;
   dg.w2 = T.w2;
   dg.n_lo2 = T.n_lo2;
-  return ctx.groups[w] = dg;
+  return ctx.groups[key] = dg;
 }
 
 const DevSchedule& dev_schedule(Context& ctx, const DevGroups& gr, int n_waves, int W) {
@@ -728,7 +732,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
   std::vector<PlannedPass> passes =
       plan_passes(rot.steps, norb, std::max(w, 1), tstore, -1);
   const DevStrings& ds = dev_strings(ctx, norb, nocc);
-  const DevGroups& gr = dev_groups(ctx, std::max(w, 1));
+  const DevGroups& gr = dev_groups(ctx, std::max(w, 1), dtype == FFSIM_C64);
   const bool need_diag = !rot.trivial_diag;
   require(!need_diag || norb <= kMaxDiagOrbitals, FFSIM_ERR_VALUE, "norb above 64");
   size_t diag_off = 0;

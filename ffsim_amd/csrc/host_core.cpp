@@ -145,7 +145,7 @@ PairTables make_pair_tables(int w) {
   return T;
 }
 
-GroupTables make_group_tables(int w, int w2) {
+GroupTables make_group_tables(int w, int w2, bool merge_complements) {
   GroupTables T;
   T.w = w;
   T.w2 = std::min(w2, w);
@@ -163,26 +163,40 @@ GroupTables make_group_tables(int w, int w2) {
         uint64_t o = s & ~sub_mask;
         if (std::find(outs.begin(), outs.end(), o) == outs.end()) outs.push_back(o);
       }
+      // A group with k2 > w2 / 2 electrons in the sub-window is stored as the
+      // complement of a class-(w2 - k2) group: member m is the string whose
+      // sub-window bits are the complement of the m-th (w2 - k2)-subset.  A
+      // rotation then pairs the same members with the roles of x and y swapped,
+      // which the kernel absorbs by negating Re(sigma) for that lane, so groups
+      // of k2 and w2 - k2 share one code path and can share a wave.
       struct G {
         uint64_t out;
-        int k2;
+        int cls;
+        bool flip;
       };
       std::vector<G> gs;
       for (uint64_t o : outs) {
         int k2 = kp - __builtin_popcountll(o);
-        if (k2 > 0 && k2 < T.w2) gs.push_back({o, k2});
+        if (k2 > 0 && k2 < T.w2) {
+          const bool flip = merge_complements && 2 * k2 > T.w2;
+          gs.push_back({o, flip ? T.w2 - k2 : k2, flip});
+        }
       }
       std::stable_sort(gs.begin(), gs.end(), [&](const G& a, const G& b) {
-        const uint64_t sa = binom(T.w2, a.k2), sb = binom(T.w2, b.k2);
-        return sa != sb ? sa > sb : a.k2 < b.k2;
+        const uint64_t sa = binom(T.w2, a.cls), sb = binom(T.w2, b.cls);
+        if (sa != sb) return sa > sb;
+        if (a.cls != b.cls) return a.cls < b.cls;
+        return a.flip < b.flip;
       });
       const int off = (int)T.desc.size();
       const int moff0 = (int)T.members.size();
       for (const G& g : gs) {
         const uint32_t moff = (uint32_t)T.members.size();
-        for (uint64_t sl : make_strings(T.w2, g.k2))
-          T.members.push_back((uint16_t)string_rank(g.out | (sl << lo2)));
-        T.desc.push_back((moff << 8) | (uint32_t)g.k2);
+        for (uint64_t sl : make_strings(T.w2, g.cls)) {
+          const uint64_t bits = g.flip ? (~sl & sub_bits) : sl;
+          T.members.push_back((uint16_t)string_rank(g.out | (bits << lo2)));
+        }
+        T.desc.push_back((moff << 8) | (g.flip ? 0x80u : 0u) | (uint32_t)g.cls);
       }
       T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 0] = off;
       T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 1] = (int)gs.size();
@@ -208,9 +222,9 @@ WaveSchedule make_wave_schedule(const GroupTables& T, int n_waves, int per_chunk
       };
       std::vector<Ch> chs;
       for (int g = 0; g < nd;) {
-        const int k2 = T.desc[d0 + g] & 0xff;
+        const int k2 = T.desc[d0 + g] & 0x7f;  // group class (flag bit 7: complemented)
         int e = g;
-        while (e < nd && e - g < per_chunk && (int)(T.desc[d0 + e] & 0xff) == k2) ++e;
+        while (e < nd && e - g < per_chunk && (int)(T.desc[d0 + e] & 0x7f) == k2) ++e;
         // rotation work per lane ~ pairs per pair position; plus the group's loads/stores
         const double pairs = (double)binom(T.w2 - 2, k2 - 1);
         const double size = (double)binom(T.w2, k2);

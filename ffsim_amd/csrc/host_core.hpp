@@ -63,24 +63,27 @@ PairTables make_pair_tables(int w);
 // Register groups of a sub-window [lo2, lo2 + w2) inside a window of width w:
 // for every kp and every sub-window start lo2 in [0, w - w2], the window-local
 // strings with kp electrons are split into groups that agree outside the
-// sub-window.  A group with k2 electrons inside the sub-window has C(w2, k2)
-// members, listed in ascending order of their sub-window bits, so member m is
-// the m-th k2-subset of w2 bits.  Groups with k2 == 0 or k2 == w2 are not
+// sub-window.  A group with k2 electrons inside the sub-window is of class k2: C(w2, k2) members in ascending order of their sub-window bits, so
+// member m is the m-th k2-subset of w2 bits.  With merge_complements, a group
+// with k2 > w2 / 2 is of class w2 - k2, complemented (descriptor bit 7): member
+// m has the complement of the m-th (w2 - k2)-subset.  Groups with k2 == 0 or k2 == w2 are not
 // stored (no rotation inside the sub-window changes them).  Groups are sorted
-// by decreasing size.
+// by decreasing size, then class, plain before complemented.
 struct GroupTables {
   int w = 0, w2 = 0, n_lo2 = 0;
-  std::vector<uint32_t> desc;     // member_offset << 8 | k2
+  std::vector<uint32_t> desc;     // member_offset << 8 | complemented << 7 | class
   std::vector<uint16_t> members;  // window-local ranks
   // [(kp * n_lo2 + lo2) * 4 + {0: desc offset, 1: group count, 2: member offset,
   //                           3: member count}]; a (kp, lo2) set is contiguous
   std::vector<int32_t> index;
 };
-GroupTables make_group_tables(int w, int w2);
+// merge_complements: store groups with k2 > w2 / 2 as complemented groups of
+// class w2 - k2 (otherwise every group is of class k2, never complemented).
+GroupTables make_group_tables(int w, int w2, bool merge_complements);
 
 // Static wave schedule of the register groups: for every (kp, lo2) the
 // groups are cut into chunks of at most `per_chunk` consecutive groups with
-// the same k2 (one chunk fills a wave: per_chunk groups x W batch columns),
+// the same class (one chunk fills a wave: per_chunk groups x W batch columns),
 // and the chunks are assigned to the block's waves longest-first, balancing
 // the rotation work per SIMD (wave i runs on SIMD i % 4).
 struct WaveSchedule {

@@ -110,6 +110,24 @@ __device__ __forceinline__ void rot_pair<double>(Cx<double>& a, Cx<double>& b, d
   asm("v_fma_f64 %0, %1, %0, -%2" : "+v"(b.im) : "s"(c), "v"(ui));
 }
 
+// The same with Re(sigma) in a VGPR: lanes of complemented groups run with the
+// negated value (see group_apply).
+__device__ __forceinline__ void rot_pair_vsr(Cx<double>& a, Cx<double>& b, double c, double sr,
+                                             double si) {
+  double ur, ui, t1, t2;
+  asm("v_mul_f64 %0, %1, %2" : "=v"(ur) : "v"(sr), "v"(a.re));
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(ur) : "s"(si), "v"(a.im));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(ui) : "v"(sr), "v"(a.im));
+  asm("v_fma_f64 %0, -%1, %2, %0" : "+v"(ui) : "s"(si), "v"(a.re));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(t1) : "s"(si), "v"(b.im));
+  asm("v_fma_f64 %0, %1, %2, -%0" : "+v"(t1) : "v"(sr), "v"(b.re));
+  asm("v_mul_f64 %0, %1, %2" : "=v"(t2) : "s"(si), "v"(b.re));
+  asm("v_fma_f64 %0, %1, %2, %0" : "+v"(t2) : "v"(sr), "v"(b.im));
+  asm("v_fma_f64 %0, %1, %0, %2" : "+v"(a.re) : "s"(c), "v"(t1));
+  asm("v_fma_f64 %0, %1, %0, %2" : "+v"(a.im) : "s"(c), "v"(t2));
+  asm("v_fma_f64 %0, %1, %0, -%2" : "+v"(b.re) : "s"(c), "v"(ur));
+  asm("v_fma_f64 %0, %1, %0, -%2" : "+v"(b.im) : "s"(c), "v"(ui));
+}
 // FP32 (complex64 states): the same in-place form with v_fma_f32 / v_mul_f32.
 template <>
 __device__ __forceinline__ void rot_pair<float>(Cx<float>& a, Cx<float>& b, float c, float sr,
@@ -128,6 +146,11 @@ __device__ __forceinline__ void rot_pair<float>(Cx<float>& a, Cx<float>& b, floa
   asm("v_fma_f32 %0, %1, %0, -%2" : "+v"(b.re) : "v"(c), "v"(ur));
   asm("v_fma_f32 %0, %1, %0, -%2" : "+v"(b.im) : "v"(c), "v"(ui));
 }
+// c64 passes every coefficient in a VGPR already.
+__device__ __forceinline__ void rot_pair_vsr(Cx<float>& a, Cx<float>& b, float c, float sr,
+                                             float si) {
+  rot_pair<float>(a, b, c, sr, si);
+}
 
 template <typename T, int W2, int K2, int P>
 __device__ __forceinline__ void rot_sub(Cx<T>* v, T c, T sr, T si) {
@@ -139,10 +162,31 @@ __device__ __forceinline__ void rot_sub(Cx<T>* v, T c, T sr, T si) {
 }
 
 template <typename T, int W2, int K2, int P>
-__device__ __forceinline__ void slot_apply(Cx<T>* v, const double* c, const double* sr,
-                                           const double* si, int mask) {
+__device__ __forceinline__ void rot_sub_vsr(Cx<T>* v, T c, T sr, T si) {
   if constexpr (P + 1 < W2) {
-    if (mask & (1 << P)) rot_sub<T, W2, K2, P>(v, (T)c[P], (T)sr[P], (T)si[P]);
+    constexpr SubPairList pl = make_sub_pairs<W2, K2, P>();
+#pragma unroll
+    for (int q = 0; q < pl.n; ++q) rot_pair_vsr(v[pl.x[q]], v[pl.y[q]], c, sr, si);
+  }
+}
+
+// One slot of a layer pair.  A complemented group (flip) applies the rotation
+// with Re(sigma) negated: its members pair up with x and y exchanged, and
+// x' = c x + s y, y' = c y - conj(s) x with x <-> y is the same update with
+// s -> -conj(s).  Self-complementary classes (2 K2 == W2) are never flipped,
+// and c128 tables have no complemented groups (Re(sigma) stays in SGPRs).
+template <typename T, int W2, int K2, int P>
+__device__ __forceinline__ void slot_apply(Cx<T>* v, const double* c, const double* sr,
+                                           const double* si, int mask, bool flip) {
+  if constexpr (P + 1 < W2) {
+    if (mask & (1 << P)) {
+      if constexpr (2 * K2 == W2 || sizeof(T) == 8) {
+        rot_sub<T, W2, K2, P>(v, (T)c[P], (T)sr[P], (T)si[P]);
+      } else {
+        const T s = flip ? -(T)sr[P] : (T)sr[P];
+        rot_sub_vsr<T, W2, K2, P>(v, (T)c[P], s, (T)si[P]);
+      }
+    }
   }
 }
 
@@ -150,7 +194,7 @@ __device__ __forceinline__ void slot_apply(Cx<T>* v, const double* c, const doub
 template <typename T, int W2, int K2>
 __device__ __forceinline__ void group_apply(Cx<T>* tile, int S, int j,
                                             const uint16_t* __restrict__ members,
-                                            const LayerPair* lp, int n_lp) {
+                                            const LayerPair* lp, int n_lp, bool flip) {
   constexpr int N = binom_const(W2, K2);
   Cx<T> v[N];
   int idx[N];
@@ -169,35 +213,35 @@ __device__ __forceinline__ void group_apply(Cx<T>* tile, int S, int j,
       sr[q] = L.sr[q];
       si[q] = L.si[q];
     }
-    slot_apply<T, W2, K2, 0>(v, c, sr, si, mask);
-    slot_apply<T, W2, K2, 2>(v, c, sr, si, mask);
-    slot_apply<T, W2, K2, 4>(v, c, sr, si, mask);
-    slot_apply<T, W2, K2, 1>(v, c, sr, si, mask);
-    slot_apply<T, W2, K2, 3>(v, c, sr, si, mask);
+    slot_apply<T, W2, K2, 0>(v, c, sr, si, mask, flip);
+    slot_apply<T, W2, K2, 2>(v, c, sr, si, mask, flip);
+    slot_apply<T, W2, K2, 4>(v, c, sr, si, mask, flip);
+    slot_apply<T, W2, K2, 1>(v, c, sr, si, mask, flip);
+    slot_apply<T, W2, K2, 3>(v, c, sr, si, mask, flip);
   }
 #pragma unroll
   for (int m = 0; m < N; ++m) tile[idx[m]] = v[m];
 }
 
 template <typename T, int W2>
-__device__ __forceinline__ void group_dispatch(int k2, Cx<T>* tile, int S, int j,
+__device__ __forceinline__ void group_dispatch(int k2, bool flip, Cx<T>* tile, int S, int j,
                                                const uint16_t* members, const LayerPair* lp,
                                                int n_lp) {
   switch (k2) {
     case 1:
-      if constexpr (W2 >= 2) group_apply<T, W2, 1>(tile, S, j, members, lp, n_lp);
+      if constexpr (W2 >= 2) group_apply<T, W2, 1>(tile, S, j, members, lp, n_lp, flip);
       break;
     case 2:
-      if constexpr (W2 >= 3) group_apply<T, W2, 2>(tile, S, j, members, lp, n_lp);
+      if constexpr (W2 >= 3) group_apply<T, W2, 2>(tile, S, j, members, lp, n_lp, flip);
       break;
     case 3:
-      if constexpr (W2 >= 4) group_apply<T, W2, 3>(tile, S, j, members, lp, n_lp);
+      if constexpr (W2 >= 4) group_apply<T, W2, 3>(tile, S, j, members, lp, n_lp, flip);
       break;
     case 4:
-      if constexpr (W2 >= 5) group_apply<T, W2, 4>(tile, S, j, members, lp, n_lp);
+      if constexpr (W2 >= 5) group_apply<T, W2, 4>(tile, S, j, members, lp, n_lp, flip);
       break;
     case 5:
-      if constexpr (W2 >= 6) group_apply<T, W2, 5>(tile, S, j, members, lp, n_lp);
+      if constexpr (W2 >= 6) group_apply<T, W2, 5>(tile, S, j, members, lp, n_lp, flip);
       break;
     default:
       break;
@@ -600,7 +644,12 @@ __global__ void __launch_bounds__(NTHR)
         const int gl = lane / W, j = lane % W;
         if (gl < (int)(ch & 0xff)) {
           const uint32_t d = gdesc[gs.d_off + (int)(ch >> 8) + gl];
-          group_dispatch<T, W2>(d & 0xff, tile, S, j, gmem + (d >> 8), lp, P.n_lp);
+          // c64 tables hold complemented groups (bit 7); c128 tables never do
+          if constexpr (sizeof(T) == 4)
+            group_dispatch<T, W2>(d & 0x7f, (d >> 7) & 1, tile, S, j, gmem + (d >> 8), lp,
+                                  P.n_lp);
+          else
+            group_dispatch<T, W2>(d & 0xff, false, tile, S, j, gmem + (d >> 8), lp, P.n_lp);
         }
       }
     }

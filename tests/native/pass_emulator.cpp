@@ -32,12 +32,14 @@ static void run_pass(const C* src, C* dst, long ld_src, long ld_dst, int n_batch
       for (auto& sp : subs) {
         int gi = (fd.kp * GT.n_lo2 + sp.lo2) * 4; int goff = GT.index[gi], ng = GT.index[gi + 1];
         for (int g = 0; g < ng; ++g) {
-          uint32_t d = GT.desc[goff + g]; int k2 = d & 0xff; const uint16_t* mem = GT.members.data() + (d >> 8);
+          uint32_t d = GT.desc[goff + g]; int k2 = d & 0x7f; bool flip = (d >> 7) & 1;
+          const uint16_t* mem = GT.members.data() + (d >> 8);
           int N = binom(GT.w2, k2); std::vector<C> r(N);
           for (int m = 0; m < N; ++m) r[m] = v[mem[m]];
           for (int l = 0; l < sp.n_lp; ++l) { const LayerPair& LP = lps[sp.lp0 + l];
             for (int p : {0, 2, 4, 1, 3}) { if (p + 1 >= GT.w2 || !(LP.mask & (1 << p))) continue;
-              C sg(LP.sr[p], LP.si[p]);
+              // complemented group: x and y exchanged, i.e. sigma -> -conj(sigma)
+              C sg(flip ? -LP.sr[p] : LP.sr[p], LP.si[p]);
               for (auto& xy : sub_pairs(GT.w2, k2, p)) { C a = r[xy.first], bb = r[xy.second];
                 r[xy.first] = LP.c[p] * a + sg * bb; r[xy.second] = LP.c[p] * bb - std::conj(sg) * a; } } }
           for (int m = 0; m < N; ++m) v[mem[m]] = r[m];
@@ -52,12 +54,12 @@ static void run_pass(const C* src, C* dst, long ld_src, long ld_dst, int n_batch
   }
 }
 
-static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* mat, int w, C* tdst, long ld_t) {
+static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* mat, int w, C* tdst, long ld_t, int merge) {
   std::vector<KStep> steps; std::vector<C> diag(norb, 1.0); bool has_diag = false;
   if (mat) { auto g = givens_decomposition(mat, norb, 1e-12); for (auto& r : g.rotations) steps.push_back(kernel_step_from_rotation(r)); diag = g.diag; has_diag = true; }
   auto passes = plan_passes(steps, norb, w, tdst != nullptr, -1);
   auto strings = make_strings(norb, nocc);
-  auto GT = make_group_tables(w, kMaxSubWindow);
+  auto GT = make_group_tables(w, kMaxSubWindow, merge != 0);
   for (auto& P : passes) {
     auto L = make_window_layout(norb, nocc, P.lo, w);
     bool ts = P.last && tdst; bool dg = P.last && has_diag;
@@ -65,9 +67,9 @@ static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* 
   }
 }
 
-extern "C" void emulate(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w) {
+extern "C" void emulate(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w, int merge) {
   long da = binom(norb, na), db = binom(norb, nb);
   C* v = (C*)vec; C* wk = (C*)work;
-  sector(v, db, (int)db, norb, na, (const C*)ma, w, wk, da);      // alpha rows, tstore into work
-  sector(wk, da, (int)da, norb, nb, (const C*)mb, w, v, db);      // beta rows on transposed, tstore back
+  sector(v, db, (int)db, norb, na, (const C*)ma, w, wk, da, merge);  // alpha rows, tstore into work
+  sector(wk, da, (int)da, norb, nb, (const C*)mb, w, v, db, merge);  // beta rows on transposed, tstore back
 }

@@ -34,15 +34,19 @@ def emulator(tmp_path_factory):
         check=True, capture_output=True)
     lib = ctypes.CDLL(str(out))
     P = ctypes.c_void_p
-    lib.emulate.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int]
+    lib.emulate.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int,
+                            ctypes.c_int]
     return lib
 
 
+@pytest.mark.parametrize("merge", [0, 1])
 @pytest.mark.parametrize("norb,nelec,w", [
     (4, (2, 2), 4), (5, (2, 3), 5), (6, (3, 2), 4), (7, (0, 3), 4), (7, (7, 2), 4),
     (8, (4, 4), 5), (8, (3, 5), 6), (9, (2, 7), 5), (10, (5, 4), 6), (12, (6, 6), 8),
     (12, (6, 6), 12), (13, (6, 5), 12)])
-def test_emulated_passes_match_oracle(emulator, norb, nelec, w, rng):
+def test_emulated_passes_match_oracle(emulator, norb, nelec, w, merge, rng):
+    """merge=1: groups with more than half the sub-window occupied are stored as
+    complemented groups of the mirrored class (the c64 tables)."""
     da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
     vec = O.random_state_vector(da * db, rng)
     Ua, Ub = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
@@ -51,5 +55,5 @@ def test_emulated_passes_match_oracle(emulator, norb, nelec, w, rng):
     work = np.zeros_like(x)
     p = lambda a: np.ascontiguousarray(a).ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     ua, ub = np.ascontiguousarray(Ua), np.ascontiguousarray(Ub)
-    emulator.emulate(p(x), p(work), norb, nelec[0], nelec[1], p(ua), p(ub), w)
+    emulator.emulate(p(x), p(work), norb, nelec[0], nelec[1], p(ua), p(ub), w, merge)
     np.testing.assert_allclose(x, expected, atol=1e-12, rtol=0)

@@ -2,9 +2,13 @@
 //
 // fiber_kernel: applies a program of Givens rotations (and optionally the
 //   diagonal phases of an orbital rotation) to every fiber of a window.  One
-//   workgroup owns one fiber x W batch vectors, staged in LDS; rotations run
-//   one after another on the LDS tile (a rotation pairs local strings x, y that
-//   differ by moving one electron between adjacent window orbitals).
+//   workgroup owns one fiber x W batch vectors, staged in LDS; the rotations
+//   run as register sub-passes: each lane loads one group of strings that
+//   agree outside a 6-orbital sub-window, applies that sub-window's rotations
+//   in registers and writes the group back (a rotation pairs local strings
+//   x, y that differ by moving one electron between adjacent orbitals).  The
+//   last pass of a sector can store the state transposed and, with the
+//   fused-diagonal instantiation, multiply by a diagonal Coulomb evolution.
 // diag_kernel: one elementwise pass over the state for a diagonal operator
 //   E(a, b) = row term + column term + cross term, used for diag-Coulomb and
 //   number-operator evolution (psi *= exp(-i t E)) and contraction

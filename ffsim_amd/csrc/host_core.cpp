@@ -188,14 +188,58 @@ GroupTables make_group_tables(int w, int w2, bool merge_complements) {
         if (a.cls != b.cls) return a.cls < b.cls;
         return a.flip < b.flip;
       });
+      auto member_ranks = [&](const G& g) {
+        std::vector<uint16_t> r;
+        for (uint64_t sl : make_strings(T.w2, g.cls)) {
+          const uint64_t bits = g.flip ? (~sl & sub_bits) : sl;
+          r.push_back((uint16_t)string_rank(g.out | (bits << lo2)));
+        }
+        return r;
+      };
+      if (!merge_complements) {
+        // c128 tiles hold one string per 128 bytes, and an LDS access serves the
+        // strings of two neighbouring lane groups together: pair each group with
+        // one whose member ranks differ in parity for as many members as
+        // possible, so the two strings sit on different bank halves.
+        std::vector<G> paired;
+        paired.reserve(gs.size());
+        for (size_t i = 0; i < gs.size();) {
+          size_t j = i;
+          while (j < gs.size() && gs[j].cls == gs[i].cls && gs[j].flip == gs[i].flip) ++j;
+          std::vector<G> left(gs.begin() + i, gs.begin() + j);
+          std::vector<std::vector<uint16_t>> ranks;
+          for (const G& g : left) ranks.push_back(member_ranks(g));
+          std::vector<bool> used(left.size(), false);
+          for (size_t a = 0; a < left.size(); ++a) {
+            if (used[a]) continue;
+            used[a] = true;
+            paired.push_back(left[a]);
+            size_t best = left.size();
+            int best_same = 1 << 30;
+            for (size_t b = a + 1; b < left.size(); ++b) {
+              if (used[b]) continue;
+              int same = 0;
+              for (size_t m = 0; m < ranks[a].size(); ++m)
+                same += (ranks[a][m] & 1) == (ranks[b][m] & 1);
+              if (same < best_same) {
+                best_same = same;
+                best = b;
+              }
+            }
+            if (best < left.size()) {
+              used[best] = true;
+              paired.push_back(left[best]);
+            }
+          }
+          i = j;
+        }
+        gs.swap(paired);
+      }
       const int off = (int)T.desc.size();
       const int moff0 = (int)T.members.size();
       for (const G& g : gs) {
         const uint32_t moff = (uint32_t)T.members.size();
-        for (uint64_t sl : make_strings(T.w2, g.cls)) {
-          const uint64_t bits = g.flip ? (~sl & sub_bits) : sl;
-          T.members.push_back((uint16_t)string_rank(g.out | (bits << lo2)));
-        }
+        for (uint16_t r : member_ranks(g)) T.members.push_back(r);
         T.desc.push_back((moff << 8) | (g.flip ? 0x80u : 0u) | (uint32_t)g.cls);
       }
       T.index[((size_t)kp * T.n_lo2 + lo2) * 4 + 0] = off;

@@ -503,24 +503,24 @@ int choose_window(Context& ctx, int norb, int nocc, int dtype) {
 }
 
 // LDS of a phase pass beyond the tile: orbital phases and one string per row,
-// and with a fused diagonal the batch columns' factors.
-size_t diag_lds_bytes(bool diag, bool fuse, int n_diag, int rows, int W, int n_fac) {
+// and with a fused diagonal the batch columns' factors and the store's tables.
+size_t diag_lds_bytes(bool diag, bool fuse, int n_diag, int rows, int W, int n_fac, int win) {
   if (!diag && !fuse) return 0;
   size_t b = 2 * kMaxDiagOrbitals * sizeof(double) + align16((size_t)rows * (n_diag <= 32 ? 4 : 8));
-  if (fuse) b += (size_t)W * n_fac * 2 * sizeof(double);
+  if (fuse)
+    b += ((size_t)W * n_fac + (size_t)fuse_nibble_entries(win, W)) * 2 * sizeof(double);
   return b;
 }
 
-// LDS of a fiber launch (the kernel's layout; the tile stride is taken padded,
-// as in column mode, so row mode may reserve a little more than it uses);
-// n_waves sizes the wave schedule.
-size_t fiber_lds_bytes(size_t elem, int W, int n_waves, int rows, int gd_cap, int gm_cap,
-                       bool diag, bool fuse, int n_diag, int n_fac) {
-  const int S = (W >= 8) ? W + 1 : W;
+// LDS of a fiber launch, following the kernel's layout (the tile stride is
+// padded in column mode only); n_waves sizes the wave schedule.
+size_t fiber_lds_bytes(size_t elem, int W, bool cols, int n_waves, int rows, int gd_cap,
+                       int gm_cap, bool diag, bool fuse, int n_diag, int n_fac, int win) {
+  const int S = (cols && W >= 8) ? W + 1 : W;
   return align16((size_t)rows * S * elem) + align16((size_t)rows * 4) +
          kMaxLoSlots * sizeof(GroupSlot) + align16((size_t)gd_cap * 4) +
          align16((size_t)gm_cap * 2) + (size_t)kMaxLoSlots * n_waves * 8 +
-         align16((size_t)gd_cap * 4) + diag_lds_bytes(diag, fuse, n_diag, rows, W, n_fac);
+         align16((size_t)gd_cap * 4) + diag_lds_bytes(diag, fuse, n_diag, rows, W, n_fac, win);
 }
 
 template <typename T, int W, int MODE, int W2, int NTHR, bool FUSE = false>
@@ -533,8 +533,9 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
   if (n_f == 0 || n_batch == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
-  const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, NTHR / 64, rows, gd_cap, gm_cap,
-                                     d_diag != nullptr, FUSE, n_diag, fz.n_fac);
+  const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, MODE == MODE_COLS, NTHR / 64, rows,
+                                     gd_cap, gm_cap, d_diag != nullptr, FUSE, n_diag, fz.n_fac,
+                                     fz.win);
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
   auto kern = fiber_kernel<T, W, MODE, NTHR, W2, FUSE>;
   const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
@@ -579,9 +580,9 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                                     d_diag, d_strings, n_diag, lds_max, stream)
   // 512 threads when one block fills the CU's LDS, 256 when two or more fit
   const int rows_l = all_fibers ? win.max_rows : win.max_rows_active;
-  const size_t lds_l = fiber_lds_bytes(sizeof(Cx<T>), W, 16, rows_l, gd_cap, gm_cap,
-                                       d_diag != nullptr, fz != nullptr, n_diag,
-                                       fz ? fz->n_fac : 0);
+  const size_t lds_l = fiber_lds_bytes(sizeof(Cx<T>), W, mode == MODE_COLS, 16, rows_l, gd_cap,
+                                       gm_cap, d_diag != nullptr, fz != nullptr, n_diag,
+                                       fz ? fz->n_fac : 0, fz ? fz->win : 0);
   const bool wide = 2 * lds_l > lds_max;
   if (fz) {
     // the fused diagonal store exists for the c128 production shape only
@@ -779,11 +780,12 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     const bool all = ts || dg;
     // fuse the diagonal into this pass when the kernel's fused layout applies
     bool fz_on = false;
+    // (the fused store takes the pass's window to be [0, w), w <= 16)
     if (ts && fuse && dtype == FFSIM_C128 && mode == MODE_ROWS && tile_width(dtype) == 8 &&
-        gr.w2 == kMaxSubWindow && n_sub > 0) {
+        gr.w2 == kMaxSubWindow && n_sub > 0 && P.lo == 0 && w <= 16) {
       const int rows = win.max_rows;
-      const size_t lds = fiber_lds_bytes(16, 8, 16, rows, gd_cap, gm_cap, dg, true, norb,
-                                         fuse->n_fac);
+      const size_t lds = fiber_lds_bytes(16, 8, false, 16, rows, gd_cap, gm_cap, dg, true, norb,
+                                         fuse->n_fac, w);
       const int nthr = 2 * lds > ctx.lds_max ? 512 : 256;
       fz_on = lds <= ctx.lds_max && rows <= 2 * nthr && 8 * fuse->n_fac <= nthr;
     }
@@ -793,7 +795,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
       DiagFuse dfz{};
       if (fz_on)
         dfz = DiagFuse{prog.dev<Cx<double>>(ft.bfac_off), prog.dev<Cx<double>>(ft.rfac_off),
-                       ft.n_fac};
+                       ft.n_fac, w};
       fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, gr, los, gd_cap,
                  gm_cap, prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff), n_layers,
                  dg ? prog.dev<double>(diag_off) : nullptr, ds.d, norb, ctx, stream,

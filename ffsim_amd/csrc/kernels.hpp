@@ -288,7 +288,15 @@ struct DiagFuse {
   const Cx<double>* bfac;  // [n_batch][n_fac]: per orbital q < norb, then the batch term
   const Cx<double>* rfac;  // [strings of the tile sector]
   int n_fac;               // norb + 1 (at most 64)
+  int win;                 // the pass's window is orbitals [0, win), win <= 16
 };
+// LDS of the fused store's per-tile tables: for every 4-orbital nibble t of the
+// window and every occupation pattern p of it, the product of the batch
+// columns' factors over p; then one factor per column for the batch term and
+// the fiber's orbitals outside the window (the same for every tile string).
+__host__ __device__ constexpr int fuse_nibble_entries(int win, int W) {
+  return (((win + 3) >> 2) * 16 + 1) * W;
+}
 
 // Element (string i, batch j) of a fiber tile lives at
 //   MODE_ROWS: state[row_table[i] * ld + b0 + j]   (strings index rows)
@@ -351,6 +359,7 @@ __global__ void __launch_bounds__(NTHR)
   Cx<double>* ffac = reinterpret_cast<Cx<double>*>(
       smem_raw + off + align16((size_t)max_rows * (narrow_str ? 4 : 8)));
   const bool have_str = diag != nullptr || FUSE;
+  Cx<double>* ntab = ffac + (FUSE ? W * fz.n_fac : 0);  // fused store tables
 
   const int tid = threadIdx.x;
   const int n_bt = (n_batch + W - 1) / W;
@@ -509,36 +518,54 @@ __global__ void __launch_bounds__(NTHR)
         }
       }
     } else if (FUSE) {
-      // orbital phases and the diagonal evolution: one product per row and batch
-      // column over the row string's occupied orbitals (8 independent chains)
+      // orbital phases and the diagonal evolution.  Tables first: products of
+      // the batch columns' factors over each window nibble's patterns, and the
+      // batch term times the outside orbitals' factors (shared by the fiber).
+      auto cmul = [](const Cx<double>& x, const Cx<double>& y) {
+        return Cx<double>{x.re * y.re - x.im * y.im, x.re * y.im + x.im * y.re};
+      };
+      const int nt = (fz.win + 3) >> 2;
+      const uint64_t outside = (narrow_str ? (uint64_t)rstr32[0] : rstr64[0]) &
+                               ~((1ull << fz.win) - 1);
+      for (int e = tid; e < (nt * 16 + 1) * W; e += NTHR) {
+        const int j = e % W, k = e / W;
+        const Cx<double>* F = ffac + j * fz.n_fac;
+        Cx<double> P{1.0, 0.0};
+        if (k < nt * 16) {
+          const int t4 = 4 * (k >> 4), pat = k & 15;
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            if (((pat >> b) & 1) && t4 + b < fz.win) P = cmul(P, F[t4 + b]);
+        } else {
+          P = F[fz.n_fac - 1];
+          uint64_t bits = outside;
+          while (bits) {
+            const int q = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            P = cmul(P, F[q]);
+          }
+        }
+        ntab[k * W + j] = P;
+      }
+      __syncthreads();
       for (int i = tid; i < nr; i += NTHR) {
         const Cx<double> r0f = i < NTHR ? rf[0] : rf[1];
-        double Pr[W], Pi[W];
+        const uint32_t sv = (uint32_t)(narrow_str ? (uint64_t)rstr32[i] : rstr64[i]);
+        Cx<double> P[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          const Cx<double> b = ffac[j * fz.n_fac + fz.n_fac - 1];
-          Pr[j] = r0f.re * b.re - r0f.im * b.im;
-          Pi[j] = r0f.re * b.im + r0f.im * b.re;
-        }
-        uint64_t bits = narrow_str ? (uint64_t)rstr32[i] : rstr64[i];
-        while (bits) {
-          const int q = __builtin_ctzll(bits);
-          bits &= bits - 1;
+        for (int j = 0; j < W; ++j) P[j] = cmul(r0f, ntab[nt * 16 * W + j]);
+        for (int t = 0; t < nt; ++t) {
+          const int pat = (sv >> (4 * t)) & 15;
 #pragma unroll
-          for (int j = 0; j < W; ++j) {
-            const Cx<double> f = ffac[j * fz.n_fac + q];
-            const double t = Pr[j] * f.re - Pi[j] * f.im;
-            Pi[j] = Pr[j] * f.im + Pi[j] * f.re;
-            Pr[j] = t;
-          }
+          for (int j = 0; j < W; ++j) P[j] = cmul(P[j], ntab[(t * 16 + pat) * W + j]);
         }
         const long r = rows[i];
 #pragma unroll
         for (int j = 0; j < W; ++j) {
           if (j < nb) {
             Cx<T> v = tile[i * S + j];
-            const T t = v.re * (T)Pr[j] - v.im * (T)Pi[j];
-            v.im = v.re * (T)Pi[j] + v.im * (T)Pr[j];
+            const T t = v.re * (T)P[j].re - v.im * (T)P[j].im;
+            v.im = v.re * (T)P[j].im + v.im * (T)P[j].re;
             v.re = t;
             dst[(long)(b0 + j) * ld_dst + r] = v;
           }

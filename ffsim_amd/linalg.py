@@ -91,3 +91,131 @@ def unitary_from_parameters(params: np.ndarray, dim: int) -> np.ndarray:
 def unitary_to_parameters(mat: np.ndarray) -> np.ndarray:
     """Parameters of logm(U) (python/ffsim/linalg/util.py:127-148)."""
     return antihermitians_to_parameters(scipy.linalg.logm(np.asarray(mat))[None])
+
+
+# ---------------------------------------------------------------------------
+# Double factorization of t2 amplitudes (the explicit, nested-eigendecomposition
+# form).  The optimized ("compressed") form needs an autodiff framework that is
+# not part of this package and raises NotImplementedError.
+
+
+def _ranked_eigh(mat: np.ndarray, tol: float):
+    """Eigenpairs of a real symmetric matrix, largest |eigenvalue| first, with the
+    tail whose summed |eigenvalue| stays below ``tol`` dropped
+    (python/ffsim/linalg/double_factorized_decomposition.py:39-53)."""
+    vals, vecs = scipy.linalg.eigh(mat)
+    order = np.argsort(np.abs(vals))[::-1]
+    vals, vecs = vals[order], vecs[:, order]
+    n_drop = int(np.searchsorted(np.cumsum(np.abs(vals[::-1])), tol))
+    return vals[: len(vals) - n_drop], vecs[:, : len(vals) - n_drop]
+
+
+def _ranked_svd(mat: np.ndarray, tol: float):
+    """Thin SVD with the tail whose summed singular values stay below ``tol``
+    dropped (double_factorized_decomposition.py:56-67)."""
+    left, sing, right = scipy.linalg.svd(mat, full_matrices=False)
+    keep = len(sing) - int(np.searchsorted(np.cumsum(sing[::-1]), tol))
+    return left[:, :keep], sing[:keep], right[:keep]
+
+
+def _excitation_matrix(vec: np.ndarray, nocc: int, norb: int) -> np.ndarray:
+    """norb x norb real matrix with ``vec[i * nvrt + a]`` at row ``nocc + a``,
+    column ``i`` and zeros elsewhere (the occupied->virtual block)."""
+    mat = np.zeros((norb, norb))
+    mat[nocc:, :nocc] = np.asarray(vec).reshape(nocc, norb - nocc).T
+    return mat
+
+
+def _quadrature(mat: np.ndarray, sign: int) -> np.ndarray:
+    """Hermitian matrix (1 - s i)/2 (M + s i M^dagger)
+    (double_factorized_decomposition.py:1216-1217)."""
+    return 0.5 * (1 - sign * 1j) * (mat + sign * 1j * mat.T.conj())
+
+
+def _check_t2_options(max_terms, optimize: bool) -> None:
+    if max_terms is not None and max_terms < 1:
+        raise ValueError(f"max_terms must be at least 1. Got {max_terms}.")
+    if optimize:
+        raise NotImplementedError(
+            "optimize=True (compressed double factorization) is not supported; "
+            "only the explicit factorization is available."
+        )
+
+
+def double_factorized_t2(t2_amplitudes: np.ndarray, *, tol: float = 1e-8,
+                         max_terms: int | None = None, optimize: bool = False,
+                         **optimize_kwargs):
+    r"""Double factorization :math:`t_{ijab} = i \sum_k \sum_{pq} Z^{(k)}_{pq}
+    U^{(k)}_{ap} U^{(k)*}_{ip} U^{(k)}_{bq} U^{(k)*}_{jq}` of real t2 amplitudes
+    (python/ffsim/linalg/double_factorized_decomposition.py:467-657).
+
+    The (ia),(jb) matrix of ``t2`` is eigendecomposed (largest |eigenvalue|
+    first, ``tol`` dropping the small tail); every kept eigenpair
+    :math:`(\lambda, v)` gives two terms, one for each quadrature
+    :math:`Q_\pm` of the excitation matrix built from :math:`v`:
+    :math:`Z = \pm\lambda\, e e^T` and :math:`U` the eigenvectors of
+    :math:`Q_\pm = U\,\mathrm{diag}(e)\,U^\dagger`.  Terms are returned in that
+    order, ``(+, -)`` per eigenpair, cut to ``max_terms``.  Returns
+    ``(diag_coulomb_mats, orbital_rotations)`` of shape ``(L, norb, norb)``.
+    ``optimize=True`` raises ``NotImplementedError``; its other keyword
+    arguments are accepted and ignored, as ffsim ignores them without it.
+    """
+    _check_t2_options(max_terms, optimize)
+    t2_amplitudes = np.asarray(t2_amplitudes)
+    nocc, _, nvrt, _ = t2_amplitudes.shape
+    norb = nocc + nvrt
+    t2_mat = t2_amplitudes.transpose(0, 2, 1, 3).reshape(nocc * nvrt, nocc * nvrt)
+    lam, vecs = _ranked_eigh(t2_mat, tol)
+    n_vecs = len(lam)
+    quads = np.zeros((n_vecs, 2, norb, norb), dtype=complex)
+    for k in range(n_vecs):
+        mat = _excitation_matrix(vecs[:, k], nocc, norb)
+        quads[k, 0] = _quadrature(mat, 1)
+        quads[k, 1] = _quadrature(mat, -1)
+    e, rotations = np.linalg.eigh(quads)
+    scale = lam[:, None] * np.array([1.0, -1.0])
+    mats = scale[:, :, None, None] * e[:, :, :, None] * e[:, :, None, :]
+    return (mats.reshape(-1, norb, norb)[:max_terms],
+            rotations.reshape(-1, norb, norb)[:max_terms])
+
+
+def double_factorized_t2_alpha_beta(t2_amplitudes: np.ndarray, *, tol: float = 1e-8,
+                                    max_terms: int | None = None, optimize: bool = False,
+                                    **optimize_kwargs):
+    r"""Double factorization of opposite-spin t2 amplitudes ``t2ab[i, j, a, b]``
+    (python/ffsim/linalg/double_factorized_decomposition.py:833-991, 1168-1213).
+
+    The (ia),(jb) matrix is factored by a thin SVD (``tol`` dropping the small
+    tail).  Every kept singular triple :math:`(\sigma, l, r)` gives four terms,
+    ``(s, m)`` for quadrature sign ``s in (+, -)`` and ``m in (0, 1)``: the
+    alpha rotation diagonalizes :math:`Q_s(L)`, the beta one
+    :math:`Q_s((-1)^m R)`, and with ``e`` the two eigenvalue lists side by side
+    the 2 norb x 2 norb matrix :math:`\tfrac12 c\,\sigma\, e e^T`,
+    :math:`c = (1, -1, -1, 1)`, is split into its alpha-alpha, alpha-beta and
+    beta-beta blocks.  Returns ``(diag_coulomb_mats, orbital_rotations)`` of
+    shape ``(L, 3, norb, norb)`` and ``(L, 2, norb, norb)``.
+    """
+    _check_t2_options(max_terms, optimize)
+    t2_amplitudes = np.asarray(t2_amplitudes)
+    nocc_a, nocc_b, nvrt_a, nvrt_b = t2_amplitudes.shape
+    norb = nocc_a + nvrt_a
+    t2_mat = t2_amplitudes.transpose(0, 2, 1, 3).reshape(nocc_a * nvrt_a, nocc_b * nvrt_b)
+    left, sing, right = _ranked_svd(t2_mat, tol)
+    n_vecs = len(sing)
+    quads = np.zeros((n_vecs, 2, 2, 2, norb, norb), dtype=complex)
+    for k in range(n_vecs):
+        mat_l = _excitation_matrix(left[:, k], nocc_a, norb)
+        mat_r = _excitation_matrix(right[k], nocc_b, norb)
+        for si, sign in enumerate((1, -1)):
+            quads[k, si, :, 0] = _quadrature(mat_l, sign)
+            quads[k, si, 0, 1] = _quadrature(mat_r, sign)
+            quads[k, si, 1, 1] = _quadrature(-mat_r, sign)
+    quads = quads.reshape(n_vecs, 4, 2, norb, norb)
+    e, rotations = np.linalg.eigh(quads)
+    e = np.concatenate([e[:, :, 0], e[:, :, 1]], axis=-1)
+    scale = 0.5 * np.array([1.0, -1.0, -1.0, 1.0]) * sing[:, None]
+    big = scale[:, :, None, None] * e[:, :, :, None] * e[:, :, None, :]
+    mats = np.stack([big[:, :, :norb, :norb], big[:, :, :norb, norb:],
+                     big[:, :, norb:, norb:]], axis=2)
+    return (mats.reshape(-1, 3, norb, norb)[:max_terms],
+            rotations.reshape(-1, 2, norb, norb)[:max_terms])

@@ -62,6 +62,28 @@ def random_real_symmetric_matrix(dim: int, *, rank: int | None = None, seed=None
     return mat @ mat.T
 
 
+def random_t2_amplitudes(norb: int, nocc: int, *, seed=None, dtype=complex) -> np.ndarray:
+    """random.py:240-276: one N(0,1) draw per unordered pair of excitations
+    ``(i, a) <= (j, b)`` (pairs ranked occupied-major), stored with the
+    symmetry ``t2[i, j, a, b] == t2[j, i, b, a]``; for a complex dtype the
+    tensor is then rotated by a Haar unitary drawn next from the same stream."""
+    rng = np.random.default_rng(seed)
+    nvrt = norb - nocc
+    t2 = np.zeros((nocc, nocc, nvrt, nvrt), dtype=dtype)
+    exc = [(i, a) for i in range(nocc) for a in range(nvrt)]
+    for p in range(len(exc)):
+        i, a = exc[p]
+        for j, b in exc[p:]:
+            t2[i, j, a, b] = t2[j, i, b, a] = rng.standard_normal()
+    if np.issubdtype(dtype, np.complexfloating):
+        full = np.zeros((norb,) * 4, dtype=dtype)
+        full[:nocc, :nocc, nocc:, nocc:] = t2
+        u = random_unitary(norb, seed=rng)
+        full = np.einsum("ijab,iI,jJ,aA,bB->IJAB", full, u.conj(), u.conj(), u, u)
+        t2 = full[:nocc, :nocc, nocc:, nocc:]
+    return t2
+
+
 def _symmetric_uniform(dim: int, *, mean: float, scale: float, seed=None) -> np.ndarray:
     """random.py:535-546."""
     rng = np.random.default_rng(seed)

@@ -95,6 +95,17 @@ class UCJAnglesOpSpinBalanced:
         return np.concatenate(parts) if parts else np.zeros(0)
 
     @staticmethod
+    def from_t_amplitudes(t2: np.ndarray, *, t1: np.ndarray | None = None,
+                          n_reps: int | None = None, interaction_pairs=None,
+                          tol: float = 1e-8) -> "UCJAnglesOpSpinBalanced":
+        """The spin-balanced UCJ operator from t2 (and t1) amplitudes, in angle form
+        (ucj_angles_spin_balanced.py:143-158)."""
+        from ffsim_amd.variational.ucj_spin_balanced import UCJOpSpinBalanced
+
+        return UCJAnglesOpSpinBalanced.from_ucj_op(UCJOpSpinBalanced.from_t_amplitudes(
+            t2, t1=t1, n_reps=n_reps, interaction_pairs=interaction_pairs, tol=tol))
+
+    @staticmethod
     def from_ucj_op(ucj_op) -> "UCJAnglesOpSpinBalanced":
         """From a matrix-based UCJOpSpinBalanced (ucj_angles_spin_balanced.py:161-181)."""
         num_num = [NumNumAnsatzOpSpinBalanced.from_diag_coulomb_mats(m)

@@ -144,6 +144,38 @@ class UCJOpSpinBalanced:
             final_orbital_rotation=final,
         )
 
+    @staticmethod
+    def from_t_amplitudes(t2: np.ndarray, *, t1: np.ndarray | None = None,
+                          n_reps: int | None = None, interaction_pairs=None,
+                          tol: float = 1e-8, optimize: bool = False,
+                          **optimize_kwargs) -> "UCJOpSpinBalanced":
+        """Operator from t2 (and optionally t1) amplitudes
+        (ucj_spin_balanced.py:377-540).
+
+        The repetitions are the terms of the explicit double factorization of
+        ``t2`` (same matrix for alpha-alpha and alpha-beta), cut to ``n_reps``
+        or padded with identities up to it; ``t1`` gives the final orbital
+        rotation; entries outside ``interaction_pairs`` are zeroed.
+        ``optimize=True`` raises ``NotImplementedError``.
+        """
+        from ffsim_amd.variational import util
+
+        if isinstance(n_reps, int) and n_reps <= 0:
+            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+        pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
+        _validate_interaction_pairs(pairs_aa, ordered=False)
+        _validate_interaction_pairs(pairs_ab, ordered=False)
+        mats, rots = _linalg.double_factorized_t2(t2, tol=tol, max_terms=n_reps,
+                                                  optimize=optimize, **optimize_kwargs)
+        norb = rots.shape[-1]
+        mats, rots = util.pad_reps(np.stack([mats, mats], axis=1), rots, n_reps)
+        for kind, pairs in enumerate((pairs_aa, pairs_ab)):
+            if pairs is not None:
+                mats[:, kind] *= util.pair_mask(norb, pairs, symmetric=True)
+        final = None if t1 is None else util.orbital_rotation_from_t1_amplitudes(t1)
+        return UCJOpSpinBalanced(diag_coulomb_mats=mats, orbital_rotations=rots,
+                                 final_orbital_rotation=final)
+
     def to_parameters(self, *, interaction_pairs=None) -> np.ndarray:
         """Real parameter vector (ucj_spin_balanced.py:296-373)."""
         n_reps, _, norb, _ = self.diag_coulomb_mats.shape

@@ -158,6 +158,100 @@ class UCJOpSpinUnbalanced:
             diag_coulomb_mats=mats, orbital_rotations=rots, final_orbital_rotation=final
         )
 
+    @staticmethod
+    def from_t_amplitudes(t2, *, t1=None, n_reps: int | tuple[int, int] | None = None,
+                          interaction_pairs=None, tol: float = 1e-8, optimize: bool = False,
+                          **optimize_kwargs) -> "UCJOpSpinUnbalanced":
+        """Operator from ``t2 = (t2aa, t2ab, t2bb)`` (and ``t1 = (t1a, t1b)``)
+        amplitudes (ucj_spin_unbalanced.py:430-703).
+
+        The alpha-beta repetitions come first: the terms of the opposite-spin
+        double factorization of ``t2ab``.  The same-spin ones follow, term ``k``
+        pairing the ``k``-th terms of the factorizations of ``t2aa`` and
+        ``t2bb`` (a missing one is the identity).  An int ``n_reps`` caps the
+        total, alpha-beta terms first; a pair ``(n_ab, n_same)`` caps each kind
+        separately; either pads with identities.  ``optimize=True`` raises
+        ``NotImplementedError``.
+        """
+        from ffsim_amd.variational import util
+
+        if isinstance(n_reps, int) and n_reps <= 0:
+            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+        pairs_aa, pairs_ab, pairs_bb = (
+            interaction_pairs if interaction_pairs is not None else (None, None, None)
+        )
+        _validate_interaction_pairs(pairs_aa, ordered=False)
+        _validate_interaction_pairs(pairs_ab, ordered=True)
+        _validate_interaction_pairs(pairs_bb, ordered=False)
+        t2aa, t2ab, t2bb = t2
+        nocc_a, _, nvrt_a, _ = np.shape(t2ab)
+        norb = nocc_a + nvrt_a
+        df = dict(tol=tol, optimize=optimize, **optimize_kwargs)
+
+        def empty():
+            return np.empty((0, 3, norb, norb)), np.empty((0, 2, norb, norb))
+
+        if n_reps is None or isinstance(n_reps, int):
+            cap_ab = n_reps
+        else:
+            cap_ab = n_reps[0]
+        if cap_ab is None or cap_ab >= 1:
+            mats_ab, rots_ab = _linalg.double_factorized_t2_alpha_beta(
+                t2ab, max_terms=cap_ab, **df)
+        elif cap_ab == 0:
+            mats_ab, rots_ab = empty()
+        else:
+            raise ValueError(
+                f"Number of alpha-beta repetitions must be at least 1. Got {cap_ab}.")
+
+        if n_reps is None:
+            cap_same = None
+        elif isinstance(n_reps, int):
+            cap_same = n_reps - len(mats_ab)
+        else:
+            cap_same = n_reps[1]
+        if cap_same is None or cap_same >= 1:
+            mats_aa, rots_aa = _linalg.double_factorized_t2(t2aa, max_terms=cap_same, **df)
+            mats_bb, rots_bb = _linalg.double_factorized_t2(t2bb, max_terms=cap_same, **df)
+            n_same = max(len(mats_aa), len(mats_bb))
+            if n_same:
+                mats_same = np.zeros((n_same, 3, norb, norb))
+                mats_same[: len(mats_aa), 0] = mats_aa
+                mats_same[: len(mats_bb), 2] = mats_bb
+                eye = np.eye(norb)
+                rots_same = np.stack([
+                    [rots_aa[k] if k < len(rots_aa) else eye,
+                     rots_bb[k] if k < len(rots_bb) else eye]
+                    for k in range(n_same)])
+            else:
+                mats_same, rots_same = empty()
+        elif cap_same == 0:
+            mats_same, rots_same = empty()
+        else:
+            raise ValueError(
+                f"Number of alpha-alpha repetitions must be at least 1. Got {cap_same}.")
+
+        if n_reps is None or isinstance(n_reps, int):
+            mats, rots = util.pad_reps(np.concatenate([mats_ab, mats_same])[:n_reps],
+                                       np.concatenate([rots_ab, rots_same])[:n_reps], n_reps)
+        else:
+            n_ab, n_same = n_reps
+            mats_ab, rots_ab = util.pad_reps(mats_ab[:n_ab], rots_ab[:n_ab], n_ab)
+            mats_same, rots_same = util.pad_reps(mats_same[:n_same], rots_same[:n_same],
+                                                 n_same)
+            mats = np.concatenate([mats_ab, mats_same])
+            rots = np.concatenate([rots_ab, rots_same])
+
+        final = None
+        if t1 is not None:
+            final = np.stack([util.orbital_rotation_from_t1_amplitudes(t) for t in t1])
+        for kind, pairs, symmetric in ((0, pairs_aa, True), (1, pairs_ab, False),
+                                       (2, pairs_bb, True)):
+            if pairs is not None:
+                mats[:, kind] *= util.pair_mask(norb, pairs, symmetric=symmetric)
+        return UCJOpSpinUnbalanced(diag_coulomb_mats=mats, orbital_rotations=rots,
+                                   final_orbital_rotation=final)
+
     def to_parameters(self, interaction_pairs=None) -> np.ndarray:
         """ucj_spin_unbalanced.py:331-428."""
         n_reps, _, norb, _ = self.diag_coulomb_mats.shape

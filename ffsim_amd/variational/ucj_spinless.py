@@ -116,6 +116,30 @@ class UCJOpSpinless:
         return UCJOpSpinless(diag_coulomb_mats=mats, orbital_rotations=rots,
                              final_orbital_rotation=final)
 
+    @staticmethod
+    def from_t_amplitudes(t2: np.ndarray, *, t1: np.ndarray | None = None,
+                          n_reps: int | None = None, interaction_pairs=None,
+                          tol: float = 1e-8, optimize: bool = False,
+                          **optimize_kwargs) -> "UCJOpSpinless":
+        """Operator from t2 (and optionally t1) amplitudes (ucj_spinless.py:322-453):
+        the terms of the explicit double factorization of ``t2``, cut to
+        ``n_reps`` or padded with identities up to it, ``t1`` giving the final
+        orbital rotation, entries outside ``interaction_pairs`` zeroed.
+        ``optimize=True`` raises ``NotImplementedError``."""
+        from ffsim_amd.variational import util
+
+        if isinstance(n_reps, int) and n_reps <= 0:
+            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+        _validate_interaction_pairs(interaction_pairs, ordered=False)
+        mats, rots = _linalg.double_factorized_t2(t2, tol=tol, max_terms=n_reps,
+                                                  optimize=optimize, **optimize_kwargs)
+        mats, rots = util.pad_reps(mats, rots, n_reps)
+        if interaction_pairs is not None:
+            mats *= util.pair_mask(rots.shape[-1], interaction_pairs, symmetric=True)
+        final = None if t1 is None else util.orbital_rotation_from_t1_amplitudes(t1)
+        return UCJOpSpinless(diag_coulomb_mats=mats, orbital_rotations=rots,
+                             final_orbital_rotation=final)
+
     def to_parameters(self, *, interaction_pairs=None) -> np.ndarray:
         """ucj_spinless.py:257-320."""
         n_reps, norb, _ = self.diag_coulomb_mats.shape

@@ -94,3 +94,58 @@ def test_ucj_angles_spin_balanced(norb, nelec, rng):
     np.testing.assert_allclose(
         ffsim_amd.apply_unitary(vec, g, norb, nelec),
         O.apply_orbital_rotation(vec, g.to_orbital_rotation(), norb, nelec), atol=1e-12, rtol=0)
+
+
+def _t2_real(norb, nocc, rng, scale=0.2):
+    from ffsim_amd.random import random_t2_amplitudes
+
+    return scale * random_t2_amplitudes(norb, nocc, seed=rng, dtype=float)
+
+
+@pytest.mark.parametrize("norb, nelec, n_reps", [(8, (4, 4), None), (10, (5, 5), 3)])
+def test_ucj_from_t_amplitudes(norb, nelec, n_reps, rng):
+    """Operators built from t amplitudes (eigenvector rotations, rank-one
+    diagonal Coulomb matrices) applied to the Hartree-Fock state and to a
+    random state; the angle form gives the same state as the matrix form."""
+    import ffsim_amd
+
+    _case("ucj from t amplitudes", norb, nelec, n_reps)
+    nocc = nelec[0]
+    t2 = _t2_real(norb, nocc, rng)
+    t1 = 0.1 * rng.standard_normal((nocc, norb - nocc))
+    op = ffsim_amd.UCJOpSpinBalanced.from_t_amplitudes(t2, t1=t1, n_reps=n_reps)
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    hf = np.zeros(dim, dtype=complex)
+    hf[0] = 1
+    for vec in (hf, O.random_state_vector(dim, rng)):
+        got = ffsim_amd.apply_unitary(vec, op, norb, nelec)
+        expected = O.apply_ucj_spin_balanced(vec, op.diag_coulomb_mats, op.orbital_rotations,
+                                             op.final_orbital_rotation, norb, nelec)
+        np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)
+    angles = ffsim_amd.UCJAnglesOpSpinBalanced.from_t_amplitudes(t2, t1=t1, n_reps=2)
+    matrix = ffsim_amd.UCJOpSpinBalanced.from_t_amplitudes(t2, t1=t1, n_reps=2)
+    np.testing.assert_allclose(ffsim_amd.apply_unitary(vec, angles, norb, nelec),
+                               ffsim_amd.apply_unitary(vec, matrix, norb, nelec),
+                               atol=1e-10, rtol=0)
+
+
+def test_ucj_spin_unbalanced_from_t_amplitudes(rng):
+    import ffsim_amd
+
+    norb, nelec = 7, (4, 3)
+    _case("ucj spin-unbalanced from t amplitudes", norb, nelec)
+    t2aa = _t2_real(norb, nelec[0], rng)
+    t2aa = t2aa - t2aa.transpose(1, 0, 2, 3)
+    t2bb = _t2_real(norb, nelec[1], rng)
+    t2bb = t2bb - t2bb.transpose(1, 0, 2, 3)
+    t2ab = 0.2 * rng.standard_normal((nelec[0], nelec[1], norb - nelec[0], norb - nelec[1]))
+    t1 = (0.1 * rng.standard_normal((nelec[0], norb - nelec[0])),
+          0.1 * rng.standard_normal((nelec[1], norb - nelec[1])))
+    op = ffsim_amd.UCJOpSpinUnbalanced.from_t_amplitudes((t2aa, t2ab, t2bb), t1=t1,
+                                                         n_reps=(4, 2))
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    vec = O.random_state_vector(dim, rng)
+    got = ffsim_amd.apply_unitary(vec, op, norb, nelec)
+    expected = O.apply_ucj_spin_unbalanced(vec, op.diag_coulomb_mats, op.orbital_rotations,
+                                           op.final_orbital_rotation, norb, nelec)
+    np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)

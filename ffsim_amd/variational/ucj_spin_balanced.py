@@ -176,6 +176,41 @@ class UCJOpSpinBalanced:
         return UCJOpSpinBalanced(diag_coulomb_mats=mats, orbital_rotations=rots,
                                  final_orbital_rotation=final)
 
+    @staticmethod
+    def from_cisd_vec(cisd_vec: np.ndarray, *, norb: int, nocc: int,
+                      c0_threshold: float = 1e-12, n_reps: int | None = None,
+                      interaction_pairs=None, tol: float = 1e-8, optimize: bool = False,
+                      **optimize_kwargs) -> "UCJOpSpinBalanced":
+        r"""Operator from a restricted CISD vector (ucj_spin_balanced.py:534-655).
+
+        The vector holds :math:`c_0`, then the singles ``c1[i, a]`` and then the
+        doubles ``c2[i, j, a, b]``, each flattened row-major (the layout of
+        pyscf 2.14.0 ``ci.cisd.cisdvec_to_amplitudes`` for RCISD).  With
+        :math:`t_1 = c_1 / c_0` and
+        :math:`t_{2,ijab} = c_{2,ijab} / c_0 - t_{1,ia} t_{1,jb} / 2` the
+        result is :meth:`from_t_amplitudes`.  Raises ``ValueError`` when
+        :math:`|c_0|` is within ``c0_threshold`` of zero or the length does not
+        fit ``norb`` and ``nocc``.
+        """
+        vec = np.asarray(cisd_vec)
+        nvrt = norb - nocc
+        n1 = nocc * nvrt
+        if vec.ndim != 1 or vec.shape[0] != 1 + n1 + n1 * n1:
+            raise ValueError(
+                f"A CISD vector for norb={norb}, nocc={nocc} has {1 + n1 + n1 * n1} "
+                f"entries. Got shape {vec.shape}.")
+        c0 = vec[0]
+        if abs(c0) <= c0_threshold:
+            raise ValueError(
+                f"CISD reference coefficient c0={c0} is smaller than the specified "
+                f"threshold, c0_tol={c0_threshold}.")
+        t1 = vec[1 : 1 + n1].reshape(nocc, nvrt) / c0
+        t2 = vec[1 + n1 :].reshape(nocc, nocc, nvrt, nvrt) / c0
+        t2 = t2 - 0.5 * t1[:, None, :, None] * t1[None, :, None, :]
+        return UCJOpSpinBalanced.from_t_amplitudes(
+            t2, t1=t1, n_reps=n_reps, interaction_pairs=interaction_pairs, tol=tol,
+            optimize=optimize, **optimize_kwargs)
+
     def to_parameters(self, *, interaction_pairs=None) -> np.ndarray:
         """Real parameter vector (ucj_spin_balanced.py:296-373)."""
         n_reps, _, norb, _ = self.diag_coulomb_mats.shape

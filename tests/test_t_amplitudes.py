@@ -329,3 +329,31 @@ def test_ucj_angles_from_t_amplitudes_matches_matrix_form():
     ref = UCJAnglesOpSpinBalanced.from_ucj_op(
         UCJOpSpinBalanced.from_t_amplitudes(t2, t1=t1, n_reps=3))
     np.testing.assert_array_equal(op.to_parameters(), ref.to_parameters())
+
+
+def test_ucj_from_cisd_vec_matches_t_amplitudes():
+    """ucj_spin_balanced_test.py:288-316 with a synthetic CISD vector laid out as
+    the reference documents (c0, singles, doubles; ucj_spin_balanced.py:566-568);
+    the conversion to t amplitudes is written out element by element."""
+    norb, nocc = 5, 2
+    nvrt = norb - nocc
+    c0 = 0.9
+    c1 = 0.1 * RNG.standard_normal((nocc, nvrt))
+    c2 = 0.1 * random_t2_amplitudes(norb, nocc, seed=RNG, dtype=float)
+    vec = np.concatenate([[c0], c1.ravel(), c2.ravel()])
+    before = vec.copy()
+    t1 = c1 / c0
+    t2 = np.zeros_like(c2)
+    for i, j, a, b in itertools.product(range(nocc), range(nocc), range(nvrt), range(nvrt)):
+        t2[i, j, a, b] = c2[i, j, a, b] / c0 - 0.5 * t1[i, a] * t1[j, b]
+    op = UCJOpSpinBalanced.from_cisd_vec(vec, norb=norb, nocc=nocc, n_reps=4)
+    ref = UCJOpSpinBalanced.from_t_amplitudes(t2, t1=t1, n_reps=4)
+    np.testing.assert_allclose(op.diag_coulomb_mats, ref.diag_coulomb_mats, atol=1e-12)
+    np.testing.assert_allclose(op.orbital_rotations, ref.orbital_rotations, atol=1e-12)
+    np.testing.assert_allclose(op.final_orbital_rotation, ref.final_orbital_rotation,
+                               atol=1e-12)
+    np.testing.assert_array_equal(vec, before)
+    with pytest.raises(ValueError, match="c0"):
+        UCJOpSpinBalanced.from_cisd_vec(np.concatenate([[0.0], vec[1:]]), norb=norb, nocc=nocc)
+    with pytest.raises(ValueError, match="entries"):
+        UCJOpSpinBalanced.from_cisd_vec(vec[:-1], norb=norb, nocc=nocc)

@@ -1,4 +1,4 @@
-"""One-body contraction drop-ins (python/ffsim/contract/one_body.py).
+r"""One-body contraction drop-ins (python/ffsim/contract/one_body.py).
 
 ``contract_one_body`` applies :math:`\sum_{ij\sigma} M_{ij} a^\dagger_{i\sigma}
 a_{j\sigma}` for a complex M, the operator ffsim builds from four real calls

@@ -132,7 +132,23 @@ def _quadrature(mat: np.ndarray, sign: int) -> np.ndarray:
     return 0.5 * (1 - sign * 1j) * (mat + sign * 1j * mat.T.conj())
 
 
-def _check_t2_options(max_terms, optimize: bool) -> None:
+# Keyword arguments of ffsim's t2 factorizations that only the optimized form
+# reads (double_factorized_decomposition.py:467-482).
+OPTIMIZE_ARGS = frozenset({"method", "callback", "options", "regularization",
+                           "multi_stage_start", "multi_stage_step"})
+_T2_ARGS = OPTIMIZE_ARGS | {"diag_coulomb_indices", "return_optimize_result"}
+
+
+def check_optimize_kwargs(kwargs: dict, allowed=OPTIMIZE_ARGS) -> None:
+    """TypeError for a keyword argument outside ``allowed``, as a signature that
+    spelled them out would raise."""
+    unknown = sorted(set(kwargs) - set(allowed))
+    if unknown:
+        raise TypeError(f"unexpected keyword argument(s): {', '.join(unknown)}")
+
+
+def _check_t2_options(max_terms, optimize: bool, kwargs: dict) -> None:
+    check_optimize_kwargs(kwargs, _T2_ARGS)
     if max_terms is not None and max_terms < 1:
         raise ValueError(f"max_terms must be at least 1. Got {max_terms}.")
     if optimize:
@@ -157,10 +173,13 @@ def double_factorized_t2(t2_amplitudes: np.ndarray, *, tol: float = 1e-8,
     :math:`Q_\pm = U\,\mathrm{diag}(e)\,U^\dagger`.  Terms are returned in that
     order, ``(+, -)`` per eigenpair, cut to ``max_terms``.  Returns
     ``(diag_coulomb_mats, orbital_rotations)`` of shape ``(L, norb, norb)``.
-    ``optimize=True`` raises ``NotImplementedError``; its other keyword
-    arguments are accepted and ignored, as ffsim ignores them without it.
+    ``optimize=True`` raises ``NotImplementedError``; the keyword arguments
+    only the optimized form reads (``method``, ``callback``, ``options``,
+    ``diag_coulomb_indices``, ``regularization``, ``multi_stage_start``,
+    ``multi_stage_step``, ``return_optimize_result``) are accepted and
+    ignored, as ffsim ignores them without it; any other raises ``TypeError``.
     """
-    _check_t2_options(max_terms, optimize)
+    _check_t2_options(max_terms, optimize, optimize_kwargs)
     t2_amplitudes = np.asarray(t2_amplitudes)
     nocc, _, nvrt, _ = t2_amplitudes.shape
     norb = nocc + nvrt
@@ -195,7 +214,7 @@ def double_factorized_t2_alpha_beta(t2_amplitudes: np.ndarray, *, tol: float = 1
     beta-beta blocks.  Returns ``(diag_coulomb_mats, orbital_rotations)`` of
     shape ``(L, 3, norb, norb)`` and ``(L, 2, norb, norb)``.
     """
-    _check_t2_options(max_terms, optimize)
+    _check_t2_options(max_terms, optimize, optimize_kwargs)
     t2_amplitudes = np.asarray(t2_amplitudes)
     nocc_a, nocc_b, nvrt_a, nvrt_b = t2_amplitudes.shape
     norb = nocc_a + nvrt_a

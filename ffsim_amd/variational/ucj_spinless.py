@@ -128,6 +128,7 @@ class UCJOpSpinless:
         ``optimize=True`` raises ``NotImplementedError``."""
         from ffsim_amd.variational import util
 
+        _linalg.check_optimize_kwargs(optimize_kwargs)
         if isinstance(n_reps, int) and n_reps <= 0:
             raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
         _validate_interaction_pairs(interaction_pairs, ordered=False)

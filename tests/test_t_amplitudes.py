@@ -357,3 +357,18 @@ def test_ucj_from_cisd_vec_matches_t_amplitudes():
         UCJOpSpinBalanced.from_cisd_vec(np.concatenate([[0.0], vec[1:]]), norb=norb, nocc=nocc)
     with pytest.raises(ValueError, match="entries"):
         UCJOpSpinBalanced.from_cisd_vec(vec[:-1], norb=norb, nocc=nocc)
+
+
+def test_optimize_only_arguments_are_accepted_and_others_rejected():
+    t2 = random_t2_amplitudes(4, 2, seed=RNG, dtype=float)
+    a = linalg.double_factorized_t2(t2, method="BFGS", options={"maxiter": 3},
+                                    diag_coulomb_indices=[(0, 0)], return_optimize_result=True)
+    b = linalg.double_factorized_t2(t2)
+    np.testing.assert_array_equal(a[0], b[0])
+    UCJOpSpinBalanced.from_t_amplitudes(t2, n_reps=2, regularization=0.1, multi_stage_step=1)
+    with pytest.raises(TypeError, match="metod"):
+        UCJOpSpinBalanced.from_t_amplitudes(t2, metod="BFGS")
+    with pytest.raises(TypeError, match="diag_coulomb_indices"):
+        UCJOpSpinless.from_t_amplitudes(t2, diag_coulomb_indices=[(0, 0)])
+    with pytest.raises(TypeError, match="tolerance"):
+        linalg.double_factorized_t2_alpha_beta(RNG.standard_normal((2, 2, 2, 2)), tolerance=1)

@@ -1,0 +1,145 @@
+"""Invariants of the gates on the GPU that need no oracle: the checks of
+tests/python/gates/orbital_rotation_test.py:41-258 and
+tests/python/gates/num_op_sum_test.py:77-95, restated.
+
+* an orbital rotation equals exp of the one-body operator of log U, applied
+  with scipy's expm_multiply through the GPU one-body linop;
+* rotating a Slater determinant by the eigenvectors of a Hermitian h gives an
+  eigenstate of the one-body operator of h, with the summed eigenvalues;
+* rotations compose;
+* num-op-sum evolution in the eigenbasis of h is the time evolution of h;
+* copy=True leaves the input vector and matrix untouched.
+
+Every test prints the case it runs (flushed) before touching the GPU.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse.linalg
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(4, (2, 1)), (5, (2, 2)), (6, (3, 2)), (6, (0, 3)), (7, (3, 4))]
+
+
+def _case(*args):
+    print("CASE", *args, flush=True)
+    sys.stdout.flush()
+
+
+def _dim(norb, nelec):
+    return math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+
+
+def _colex(occ):
+    """Address of the string with orbitals ``occ`` occupied: sum_t C(c_t, t + 1)."""
+    return sum(math.comb(c, t + 1) for t, c in enumerate(sorted(occ)))
+
+
+def _hermitian(n, rng):
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return a + a.T.conj()
+
+
+def _slater(norb, occ_a, occ_b):
+    nb = math.comb(norb, len(occ_b))
+    vec = np.zeros(math.comb(norb, len(occ_a)) * nb, dtype=complex)
+    vec[_colex(occ_a) * nb + _colex(occ_b)] = 1
+    return vec
+
+
+@pytest.mark.parametrize("norb, nelec", CASES)
+def test_orbital_rotation_is_exp_of_one_body(norb, nelec, rng):
+    import ffsim_amd
+
+    _case("rotation = expm(one-body log U)", norb, nelec)
+    mat = O.random_unitary(norb, rng)
+    vec = O.random_state_vector(_dim(norb, nelec), rng)
+    got = ffsim_amd.apply_orbital_rotation(vec, mat, norb, nelec)
+    op = ffsim_amd.contract.one_body_linop(scipy.linalg.logm(mat), norb, nelec)
+    expected = scipy.sparse.linalg.expm_multiply(op, vec, traceA=0)
+    np.testing.assert_allclose(got, expected, atol=1e-10, rtol=0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_orbital_rotation_prepares_eigenstates(seed):
+    import ffsim_amd
+
+    rng = np.random.default_rng(1000 + seed)
+    norb = 6
+    n_a, n_b = int(rng.integers(0, norb + 1)), int(rng.integers(0, norb + 1))
+    occ_a = sorted(rng.choice(norb, n_a, replace=False).tolist())
+    occ_b = sorted(rng.choice(norb, n_b, replace=False).tolist())
+    _case("eigenstate", norb, occ_a, occ_b)
+    h = _hermitian(norb, rng)
+    eigs, vecs = scipy.linalg.eigh(h)
+    state = _slater(norb, occ_a, occ_b)
+    before = state.copy()
+    final = ffsim_amd.apply_orbital_rotation(state, vecs, norb, (n_a, n_b))
+    np.testing.assert_allclose(np.linalg.norm(final), 1.0, atol=1e-12)
+    got = ffsim_amd.contract.contract_one_body(final, h, norb, (n_a, n_b))
+    energy = eigs[occ_a].sum() + eigs[occ_b].sum()
+    np.testing.assert_allclose(got, energy * final, atol=1e-10, rtol=0)
+    np.testing.assert_array_equal(state, before)
+
+
+@pytest.mark.parametrize("norb, nelec", CASES)
+def test_orbital_rotations_compose(norb, nelec, rng):
+    import ffsim_amd
+
+    _case("compose", norb, nelec)
+    u1, u2 = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+    vec = O.random_state_vector(_dim(norb, nelec), rng)
+    got = ffsim_amd.apply_orbital_rotation(vec, u1, norb, nelec)
+    got = ffsim_amd.apply_orbital_rotation(got, u2 @ u1.T.conj(), norb, nelec)
+    expected = ffsim_amd.apply_orbital_rotation(vec, u2, norb, nelec)
+    np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize("norb, nelec", CASES)
+def test_quadratic_hamiltonian_evolution(norb, nelec, rng):
+    import ffsim_amd
+
+    _case("quadratic evolution", norb, nelec)
+    h = _hermitian(norb, rng)
+    eigs, vecs = scipy.linalg.eigh(h)
+    time = float(rng.standard_normal())
+    vec = O.random_state_vector(_dim(norb, nelec), rng)
+    got = ffsim_amd.apply_num_op_sum_evolution(vec, eigs, time, norb, nelec,
+                                               orbital_rotation=vecs)
+    op = ffsim_amd.contract.one_body_linop(h, norb, nelec)
+    expected = scipy.sparse.linalg.expm_multiply(
+        -1j * time * op, vec, traceA=-1j * time * np.sum(np.abs(h)))
+    np.testing.assert_allclose(got, expected, atol=1e-10, rtol=0)
+
+
+def test_no_side_effects_and_large_norm(rng):
+    import ffsim_amd
+
+    norb, nelec = 12, (6, 6)
+    _case("no side effects / norm", norb, nelec)
+    vec = O.random_state_vector(_dim(norb, nelec), rng)
+    mat_a, mat_b = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+    vec0, a0, b0 = vec.copy(), mat_a.copy(), mat_b.copy()
+    out = ffsim_amd.apply_orbital_rotation(vec, (mat_a, mat_b), norb, nelec)
+    np.testing.assert_allclose(np.linalg.norm(out), 1.0, atol=1e-12)
+    np.testing.assert_array_equal(vec, vec0)
+    np.testing.assert_array_equal(mat_a, a0)
+    np.testing.assert_array_equal(mat_b, b0)
+    j = rng.standard_normal((norb, norb))
+    j = j + j.T
+    j0 = j.copy()
+    out = ffsim_amd.apply_diag_coulomb_evolution(vec, j, 0.7, norb, nelec,
+                                                 orbital_rotation=mat_a)
+    np.testing.assert_allclose(np.linalg.norm(out), 1.0, atol=1e-12)
+    np.testing.assert_array_equal(vec, vec0)
+    np.testing.assert_array_equal(j, j0)
+    np.testing.assert_array_equal(mat_a, a0)

@@ -8,7 +8,10 @@ tests/python/gates/num_op_sum_test.py:77-95, restated.
   eigenstate of the one-body operator of h, with the summed eigenvalues;
 * rotations compose;
 * num-op-sum evolution in the eigenbasis of h is the time evolution of h;
-* copy=True leaves the input vector and matrix untouched.
+* copy=True leaves the input vector and matrix untouched;
+* diagonal Coulomb evolution with one matrix for every spin pair conserves
+  <S^2> (tests/python/gates/diag_coulomb_test.py:121-148), with <S^2>
+  computed here from S^2 = Sz (Sz + 1) + S- S+ over explicit determinants.
 
 Every test prints the case it runs (flushed) before touching the GPU.
 """
@@ -143,3 +146,59 @@ def test_no_side_effects_and_large_norm(rng):
     np.testing.assert_array_equal(vec, vec0)
     np.testing.assert_array_equal(j, j0)
     np.testing.assert_array_equal(mat_a, a0)
+
+
+def spin_square(vec, norb, nelec):
+    """<S^2> = Sz (Sz + 1) + |S+ psi|^2 with S+ = sum_p a+_{p alpha} a_{p beta}, on
+    determinants a+(alpha string) a+(beta string)|0> (alpha operators first):
+    a_{p beta} passes the n_a alpha operators and the beta ones below p, and
+    a+_{p alpha} the alpha ones below p."""
+    n_a, n_b = nelec
+    sz = 0.5 * (n_a - n_b)
+    if n_b == 0 or n_a == norb:
+        return sz * (sz + 1)
+    strs = lambda k: [s for s in range(1 << norb) if bin(s).count("1") == k]  # noqa: E731
+    a_src, b_src, a_dst, b_dst = strs(n_a), strs(n_b), strs(n_a + 1), strs(n_b - 1)
+    a_rank = {s: i for i, s in enumerate(a_dst)}
+    b_rank = {s: i for i, s in enumerate(b_dst)}
+    psi = np.asarray(vec).reshape(len(a_src), len(b_src))
+    out = np.zeros((len(a_dst), len(b_dst)), dtype=complex)
+    for i, sa in enumerate(a_src):
+        for j, sb in enumerate(b_src):
+            for p in range(norb):
+                if (sb >> p) & 1 and not (sa >> p) & 1:
+                    below = (1 << p) - 1
+                    sign = (-1) ** (n_a + bin(sb & below).count("1") + bin(sa & below).count("1"))
+                    out[a_rank[sa | 1 << p], b_rank[sb & ~(1 << p)]] += sign * psi[i, j]
+    return sz * (sz + 1) + float(np.vdot(out, out).real)
+
+
+def test_spin_square_known_values():
+    """The helper itself: the two-orbital singlet (|0,1> + |1,0> in the
+    alpha-first product basis) has <S^2> = 0, the matching triplet 2, and a
+    determinant n_b - n_doubly_occupied + Sz (Sz + 1)."""
+    singlet = np.array([0, 1, 1, 0]) / np.sqrt(2)
+    triplet = np.array([0, 1, -1, 0]) / np.sqrt(2)
+    assert abs(spin_square(singlet, 2, (1, 1))) < 1e-14
+    assert abs(spin_square(triplet, 2, (1, 1)) - 2) < 1e-14
+    vec = _slater(5, [0, 1, 3], [1, 4])
+    assert abs(spin_square(vec, 5, (3, 2)) - (2 - 1 + 0.5 * 1.5)) < 1e-14
+    hf = _slater(4, [0, 1], [0, 1])
+    assert abs(spin_square(hf, 4, (2, 2))) < 1e-14
+
+
+@pytest.mark.parametrize("z_rep", [False, True])
+@pytest.mark.parametrize("norb, nelec", [(4, (2, 2)), (5, (3, 2)), (5, (1, 3))])
+def test_diag_coulomb_evolution_conserves_spin_squared(norb, nelec, z_rep, rng):
+    import ffsim_amd
+
+    _case("spin square", norb, nelec, z_rep)
+    mat = rng.standard_normal((norb, norb))
+    mat = mat @ mat.T
+    rot = O.random_unitary(norb, rng)
+    vec = O.random_state_vector(_dim(norb, nelec), rng)
+    time = float(rng.uniform())
+    before = spin_square(vec, norb, nelec)
+    out = ffsim_amd.apply_diag_coulomb_evolution(vec, mat, time, norb, nelec,
+                                                 orbital_rotation=rot, z_representation=z_rep)
+    np.testing.assert_allclose(spin_square(out, norb, nelec), before, rtol=1e-10)

@@ -45,8 +45,17 @@ def test_public_api_names():
                  "hartree_fock_state", "dim", "dims"]:
         assert hasattr(ffsim_amd, name), name
     for name in ["contract_diag_coulomb", "diag_coulomb_linop", "contract_num_op_sum",
-                 "num_op_sum_linop"]:
+                 "num_op_sum_linop", "contract_one_body", "one_body_linop"]:
         assert hasattr(ffsim_amd.contract, name), name
+    for name in ["givens_decomposition", "double_factorized_t2",
+                 "double_factorized_t2_alpha_beta"]:
+        assert hasattr(ffsim_amd.linalg, name), name
+    for name in ["random_t2_amplitudes", "random_unitary", "random_ucj_op_spin_balanced"]:
+        assert hasattr(ffsim_amd.random, name), name
+    for cls in [ffsim_amd.UCJOpSpinBalanced, ffsim_amd.UCJOpSpinless,
+                ffsim_amd.UCJOpSpinUnbalanced, ffsim_amd.UCJAnglesOpSpinBalanced]:
+        assert hasattr(cls, "from_t_amplitudes"), cls
+    assert hasattr(ffsim_amd.UCJOpSpinBalanced, "from_cisd_vec")
 
 
 def test_apply_unitary_type_error():

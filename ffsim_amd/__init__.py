@@ -15,7 +15,10 @@ from ffsim_amd.gates import (
 from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedHamiltonian
 from ffsim_amd.protocols import apply_unitary, linear_operator
 from ffsim_amd.states import dim, dims, hartree_fock_state
-from ffsim_amd.trotter import simulate_trotter_double_factorized
+from ffsim_amd.trotter import (
+    simulate_trotter_diag_coulomb_split_op,
+    simulate_trotter_double_factorized,
+)
 from ffsim_amd.variational import (
     GivensAnsatzOp,
     NumNumAnsatzOpSpinBalanced,
@@ -45,5 +48,6 @@ __all__ = [
     "linalg",
     "linear_operator",
     "random",
+    "simulate_trotter_diag_coulomb_split_op",
     "simulate_trotter_double_factorized",
 ]

@@ -1,4 +1,6 @@
-"""Double-factorized Trotter evolution (python/ffsim/trotter/double_factorized.py).
+"""Trotter evolution: double-factorized (python/ffsim/trotter/double_factorized.py)
+and split-operator for diagonal Coulomb Hamiltonians
+(python/ffsim/trotter/diagonal_coulomb_split_op.py).
 
 The host builds the gate sequence of ffsim's schedule (one-body terms merged
 into the running basis as ffsim does, trotter/double_factorized.py:89-127;
@@ -16,7 +18,7 @@ import scipy.linalg
 
 from ffsim_amd import _device, _lib
 from ffsim_amd.gates.diag_coulomb import prepared_mats
-from ffsim_amd.hamiltonians import DoubleFactorizedHamiltonian
+from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedHamiltonian
 
 
 def simulate_trotter_step_iterator(n_terms: int, time: float, order: int = 0) -> Iterator[tuple[int, float]]:
@@ -88,15 +90,30 @@ def simulate_trotter_double_factorized(
     st = _device.to_device(vec, copy)
     if n_steps == 0:
         return st.restore(st.tensor)
-    t = st.tensor
+    rotations, mats, times = trotter_gate_sequence(hamiltonian, time, norb, n_steps, order)
+    t = _run_sequence(st.tensor, rotations, [np.asarray(m, dtype=float) for m in mats], times,
+                      hamiltonian.z_representation, norb, nelec)
+    return st.restore(_apply_constant(t, hamiltonian.constant, time))
+
+
+def _apply_constant(t, constant: float, time: float):
+    if constant:
+        from ffsim_amd.hamiltonians.double_factorized_hamiltonian import axpby_
+
+        axpby_(t, t, None, cmath.exp(-1j * time * constant))
+    return t
+
+
+def _run_sequence(t, rotations, mats, times, z_representation: bool, norb: int, nelec):
+    """One C-ABI call: ``rotations[k]`` then diag-Coulomb term ``k`` at
+    ``times[k]`` for every term, then the last rotation."""
     na, nb = nelec
     _device.check_size(t, norb, nelec)
-    rotations, mats, times = trotter_gate_sequence(hamiltonian, time, norb, n_steps, order)
     rot_arr = np.ascontiguousarray(np.stack(rotations), dtype=np.complex128)
     n_terms = len(mats)
     mat_arr = np.zeros((max(n_terms, 1), 3, norb, norb))
     for k, m in enumerate(mats):
-        aa, ab, bb = prepared_mats(np.asarray(m, dtype=float), norb)
+        aa, ab, bb = prepared_mats(m, norb)
         mat_arr[k, 0], mat_arr[k, 1], mat_arr[k, 2] = aa, ab, bb
     time_arr = np.ascontiguousarray(np.asarray(times if times else [0.0], dtype=float))
     work = _device.work_buffer(t, norb, na, nb)
@@ -112,12 +129,56 @@ def simulate_trotter_double_factorized(
         _device.host_ptr(rot_arr),
         _device.host_ptr(np.ascontiguousarray(mat_arr)),
         _device.host_ptr(time_arr),
-        1 if hamiltonian.z_representation else 0,
+        1 if z_representation else 0,
         _device.stream_ptr(),
     )
-    if hamiltonian.constant:
-        phase = cmath.exp(-1j * time * hamiltonian.constant)
-        from ffsim_amd.hamiltonians.double_factorized_hamiltonian import axpby_
+    return t
 
-        axpby_(t, t, None, phase)
-    return st.restore(t)
+
+def split_op_gate_sequence(hamiltonian, time: float, norb: int, n_steps: int, order: int):
+    """(rotations, diag mats, times) of the split-operator schedule
+    (trotter/diagonal_coulomb_split_op.py:87-127): the one-body term
+    exp(-i t h) is merged into the running basis, which is applied before each
+    diagonal Coulomb term (J_aa, J_ab, J_aa) and reset to the identity."""
+    rotations, mats, times = [], [], []
+    step_time = time / n_steps
+    current = np.eye(norb, dtype=complex)
+    aa, ab = (np.asarray(m, dtype=float) for m in hamiltonian.diag_coulomb_mats)
+    for _ in range(n_steps):
+        for term, t in simulate_trotter_step_iterator(2, step_time, order):
+            if term == 0:
+                current = scipy.linalg.expm(-1j * t * hamiltonian.one_body_tensor) @ current
+            else:
+                rotations.append(current)
+                mats.append((aa, ab, aa))
+                times.append(t)
+                current = np.eye(norb, dtype=complex)
+    rotations.append(current)
+    return rotations, mats, times
+
+
+def simulate_trotter_diag_coulomb_split_op(
+    vec,
+    hamiltonian: DiagonalCoulombHamiltonian,
+    time: float,
+    *,
+    norb: int,
+    nelec: tuple[int, int],
+    n_steps: int = 1,
+    order: int = 0,
+    copy: bool = True,
+):
+    """Evolve by exp(-i H time) for a diagonal Coulomb Hamiltonian with the
+    split-operator Trotter formula
+    (python/ffsim/trotter/diagonal_coulomb_split_op.py:25-84), as one gate
+    sequence on the GPU."""
+    if order < 0:
+        raise ValueError(f"order must be non-negative, got {order}.")
+    if n_steps < 0:
+        raise ValueError(f"n_steps must be non-negative, got {n_steps}.")
+    st = _device.to_device(vec, copy)
+    if n_steps == 0:
+        return st.restore(st.tensor)
+    rotations, mats, times = split_op_gate_sequence(hamiltonian, time, norb, n_steps, order)
+    t = _run_sequence(st.tensor, rotations, mats, times, False, norb, nelec)
+    return st.restore(_apply_constant(t, hamiltonian.constant, time))

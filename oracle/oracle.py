@@ -798,6 +798,31 @@ def simulate_trotter_double_factorized(vec, one_body_tensor, diag_coulomb_mats, 
     return vec
 
 
+def simulate_trotter_diag_coulomb_split_op(vec, one_body_tensor, diag_coulomb_mats, constant,
+                                           time, norb, nelec, n_steps=1, order=0):
+    """python/ffsim/trotter/diagonal_coulomb_split_op.py:25-127: the one-body
+    term is merged into the running basis, applied before each diagonal Coulomb
+    term (J_aa, J_ab, J_aa) and reset to the identity."""
+    vec = np.asarray(vec, dtype=np.complex128).copy()
+    if n_steps == 0:
+        return vec
+    aa, ab = diag_coulomb_mats
+    step_time = time / n_steps
+    current = np.eye(norb, dtype=complex)
+    for _ in range(n_steps):
+        for term, t in trotter_step_iterator(2, step_time, order):
+            if term == 0:
+                current = scipy.linalg.expm(-1j * t * one_body_tensor) @ current
+            else:
+                vec = apply_orbital_rotation(vec, current, norb, nelec, copy=False)
+                vec = apply_diag_coulomb_evolution(vec, (aa, ab, aa), t, norb, nelec, copy=False)
+                current = np.eye(norb, dtype=complex)
+    vec = apply_orbital_rotation(vec, current, norb, nelec, copy=False)
+    if constant:
+        vec = vec * np.exp(-1j * time * constant)
+    return vec
+
+
 # ---------------------------------------------------------------------------
 # Layer 2: independent closed forms
 # ---------------------------------------------------------------------------

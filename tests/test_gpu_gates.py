@@ -322,6 +322,43 @@ def test_trotter_double_factorized(norb, nelec, order, n_steps, z_rep, rng):
     np.testing.assert_allclose(got, expected, atol=1e-11, rtol=0)
 
 
+@pytest.mark.parametrize("norb,nelec,time,n_steps,order,atol",
+                         [(3, (1, 1), 0.1, 20, 0, 1e-2), (4, (2, 1), 0.1, 10, 2, 1e-3),
+                          (4, (2, 2), 0.1, 10, 1, 1e-3), (10, (5, 4), 0.1, 2, 1, None),
+                          (13, (6, 5), 0.1, 2, 0, None)])
+def test_trotter_diag_coulomb_split_op(norb, nelec, time, n_steps, order, atol, rng):
+    """tests/python/trotter/diag_coulomb_split_op_test.py:24-83: against the
+    exact propagator (expm_multiply through the GPU Hamiltonian linop) at the
+    reference's tolerances for the small cases, and against the oracle at 1e-11
+    everywhere; norb=13 runs the fused diagonal store."""
+    import scipy.sparse.linalg
+
+    import ffsim_amd
+
+    _case("split-op trotter", norb, nelec, n_steps, order)
+    h = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    h = h + h.conj().T
+    aa, ab = (m @ m.T for m in (rng.standard_normal((norb, norb)) for _ in range(2)))
+    ham = ffsim_amd.DiagonalCoulombHamiltonian(h, np.stack([aa, ab]),
+                                               constant=float(rng.uniform(-10, 10)))
+    vec = _rand(norb, nelec, rng)
+    before = vec.copy()
+    got = ffsim_amd.simulate_trotter_diag_coulomb_split_op(vec, ham, time, norb=norb,
+                                                           nelec=nelec, n_steps=n_steps,
+                                                           order=order)
+    np.testing.assert_array_equal(vec, before)
+    np.testing.assert_allclose(np.linalg.norm(got), 1.0, atol=1e-12)
+    expected = O.simulate_trotter_diag_coulomb_split_op(vec, h, (aa, ab), ham.constant, time,
+                                                        norb, nelec, n_steps=n_steps,
+                                                        order=order)
+    np.testing.assert_allclose(got, expected, atol=1e-11, rtol=0)
+    if atol is not None:
+        linop = ffsim_amd.linear_operator(ham, norb, nelec)
+        exact = scipy.sparse.linalg.expm_multiply(
+            -1j * time * linop, vec, traceA=-1j * time * np.sum(np.abs(ham.diag_coulomb_mats)))
+        np.testing.assert_allclose(got, exact, atol=atol)
+
+
 @pytest.mark.parametrize("norb,nelec", [(1, (1, 0)), (3, (2, 1)), (4, (0, 4)), (6, (3, 2)),
                                         (8, (4, 4)), (10, (5, 3)), (12, (6, 6))])
 def test_contract_one_body(norb, nelec, rng):

@@ -163,9 +163,11 @@ int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, 
                          int n_beta, double* out, void* stream);
 
 /* A sequence of rotated diagonal-Coulomb evolutions, the gate pattern shared
- * by UCJOpSpinBalanced._apply_unitary_ (ucj_spin_balanced.py:657-696) and the
+ * by UCJOpSpinBalanced._apply_unitary_ (ucj_spin_balanced.py:657-696), the
  * double-factorized Trotter step (python/ffsim/trotter/double_factorized.py:
- * 89-127): for t in 0..n_terms-1, rotate both spin sectors by
+ * 89-127) and the split-operator Trotter step
+ * (python/ffsim/trotter/diagonal_coulomb_split_op.py:87-127): for t in
+ * 0..n_terms-1, rotate both spin sectors by
  * rotations[t] then evolve by the diag-Coulomb matrices mats[t] = (aa, ab, bb)
  * for times[t]; finally rotate by rotations[n_terms].
  * rotations: (n_terms + 1) x norb x norb complex128; mats: n_terms x 3 x norb x

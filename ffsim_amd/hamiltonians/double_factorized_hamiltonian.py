@@ -50,6 +50,51 @@ class DoubleFactorizedHamiltonian:
     def norb(self) -> int:
         return self.one_body_tensor.shape[0]
 
+    def _z_correction(self):
+        r"""One-body and constant terms that move the operator between the
+        number and Z representations (double_factorized_hamiltonian.py:308-330):
+        :math:`\sum_t U_t\,\mathrm{diag}((r_t + c_t)/2)\,U_t^\dagger`, with
+        :math:`r_t, c_t` the row and column sums of :math:`J_t`, and
+        :math:`\sum_t (\mathrm{tr} J_t / 4 - \sum_{ij} J_{t,ij} / 2)`."""
+        mats = np.asarray(self.diag_coulomb_mats)
+        rots = np.asarray(self.orbital_rotations)
+        norb = self.norb
+        one_body = np.zeros((norb, norb), dtype=complex)
+        for j, u in zip(mats, rots):
+            w = 0.5 * (j.sum(axis=1) + j.sum(axis=0))
+            one_body += (u * w) @ u.conj().T
+        constant = sum(0.25 * np.trace(j) - 0.5 * j.sum() for j in mats)
+        return one_body, float(constant)
+
+    def to_z_representation(self) -> "DoubleFactorizedHamiltonian":
+        """The same operator in the Z representation
+        (double_factorized_hamiltonian.py:107-121)."""
+        if self.z_representation:
+            return self
+        one_body, constant = self._z_correction()
+        return dataclasses.replace(self, one_body_tensor=self.one_body_tensor + one_body,
+                                   constant=self.constant + constant, z_representation=True)
+
+    def to_number_representation(self) -> "DoubleFactorizedHamiltonian":
+        """The same operator in the number representation
+        (double_factorized_hamiltonian.py:123-138)."""
+        if not self.z_representation:
+            return self
+        one_body, constant = self._z_correction()
+        return dataclasses.replace(self, one_body_tensor=self.one_body_tensor - one_body,
+                                   constant=self.constant - constant, z_representation=False)
+
+    def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
+        """double_factorized_hamiltonian.py:286-305."""
+        if not isinstance(other, DoubleFactorizedHamiltonian):
+            return NotImplemented
+        return (self.z_representation == other.z_representation
+                and all(np.allclose(x, y, rtol=rtol, atol=atol) for x, y in (
+                    (self.constant, other.constant),
+                    (self.one_body_tensor, other.one_body_tensor),
+                    (self.diag_coulomb_mats, other.diag_coulomb_mats),
+                    (self.orbital_rotations, other.orbital_rotations))))
+
     def _linear_operator_(self, norb: int, nelec) -> DeviceLinearOperator:
         """Same operator as ffsim's (double_factorized_hamiltonian.py:244-275): the
         one-body term as a rotated number-operator sum, plus one rotated

@@ -246,6 +246,10 @@ def test_df_hamiltonian_linop(rng):
         expected = O.df_hamiltonian_matvec(vec, df.one_body_tensor, df.diag_coulomb_mats,
                                            df.orbital_rotations, df.constant, norb, nelec, z)
         np.testing.assert_allclose(op @ vec, expected, atol=1e-10, rtol=0)
+        # the other representation is the same operator
+        other = df.to_number_representation() if z else df.to_z_representation()
+        np.testing.assert_allclose(ffsim_amd.linear_operator(other, norb, nelec) @ vec,
+                                   expected, atol=1e-10, rtol=0)
 
 
 def test_diagonal_coulomb_hamiltonian_linop(rng):

@@ -357,3 +357,27 @@ def test_split_op_trotter_vs_exact_propagator(rng):
         got = O.simulate_trotter_diag_coulomb_split_op(vec, hh, (a, b), 0.0, t, norb, nelec)
         want = scipy.linalg.expm(-1j * t * _dense_dc(hh, a, b, 0.0, norb, nelec)) @ vec
         np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("norb, nelec", [(4, (2, 1)), (5, (2, 2))])
+def test_df_z_representation_is_the_same_operator(norb, nelec, rng):
+    """DoubleFactorizedHamiltonian.to_z_representation / to_number_representation
+    (double_factorized_hamiltonian.py:107-138, :308-330): the dense operator built
+    from the number-representation energies equals the one built from the
+    Z-representation energies after the conversion, and the round trip is exact."""
+    from ffsim_amd.hamiltonians import DoubleFactorizedHamiltonian
+
+    h = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    h = h + h.conj().T
+    mats = np.stack([O.random_symmetric_uniform(norb, rng) for _ in range(3)])
+    rots = np.stack([O.random_unitary(norb, rng) for _ in range(3)])
+    ham = DoubleFactorizedHamiltonian(h, mats, rots, constant=0.4)
+    z = ham.to_z_representation()
+    assert z.z_representation and z.to_z_representation() is z
+    np.testing.assert_allclose(
+        _dense_df(z.one_body_tensor, mats, rots, z.constant, norb, nelec, z_rep=True),
+        _dense_df(h, mats, rots, 0.4, norb, nelec, z_rep=False), atol=1e-11)
+    back = z.to_number_representation()
+    assert not back.z_representation and ham.to_number_representation() is ham
+    assert back._approx_eq_(ham, rtol=0, atol=1e-12)
+    assert not z._approx_eq_(ham, rtol=1e-5, atol=1e-8)

@@ -445,7 +445,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py runs on one GPU only: this build has no multi-GPU path "
+            f"bench.py runs on one GPU only: ffsim_amd has no multi-GPU path "
             f"(asked for --gpus {args.gpus}).",
             file=sys.stderr,
         )

@@ -35,3 +35,18 @@ def linear_operator(obj: Any, norb: int, nelec):
     if method is not None:
         return method(norb=norb, nelec=nelec)
     raise TypeError(f"Object of type {type(obj)} has no _linear_operator_ method.")
+
+
+def approx_eq(obj: Any, other: Any, rtol: float = 1e-5, atol: float = 1e-8) -> bool:
+    """Whether two objects are equal within ``rtol``/``atol`` (``np.isclose``
+    sense).  Same dispatch as ffsim.approx_eq
+    (python/ffsim/protocols/approximate_equality_protocol.py:40-70): ``obj._approx_eq_``,
+    then ``other._approx_eq_``, each skipped when missing or NotImplemented,
+    then ``==``."""
+    for a, b in ((obj, other), (other, obj)):
+        method = getattr(a, "_approx_eq_", None)
+        if method is not None:
+            result = method(b, rtol=rtol, atol=atol)
+            if result is not NotImplemented:
+                return result
+    return obj == other

@@ -167,3 +167,14 @@ def test_gate_without_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError, match="GPU"):
         ffsim_amd.apply_orbital_rotation(np.ones(4, dtype=complex) / 2, np.eye(2), 2, (1, 1))
+
+
+def test_approx_eq_protocol(rng):
+    """python/ffsim/protocols/approximate_equality_protocol.py:40-70: dispatch to
+    either side's _approx_eq_, else ==."""
+    op = ffsim_amd.random.random_ucj_op_spin_balanced(4, n_reps=2, seed=rng)
+    same = ffsim_amd.UCJOpSpinBalanced(op.diag_coulomb_mats + 1e-10, op.orbital_rotations)
+    assert ffsim_amd.approx_eq(op, same)
+    assert not ffsim_amd.approx_eq(op, same, rtol=0, atol=1e-12)
+    assert not ffsim_amd.approx_eq(op, "x") and not ffsim_amd.approx_eq("x", op)
+    assert ffsim_amd.approx_eq(3, 3) and not ffsim_amd.approx_eq(3, 4)

@@ -4,6 +4,8 @@ from __future__ import annotations
 
 from typing import Any
 
+import numpy as np
+
 
 def apply_unitary(vec, obj: Any, norb: int, nelec, copy: bool = True):
     """Apply a unitary transformation to a vector.
@@ -50,3 +52,32 @@ def approx_eq(obj: Any, other: Any, rtol: float = 1e-5, atol: float = 1e-8) -> b
             if result is not NotImplemented:
                 return result
     return obj == other
+
+
+def diag(obj: Any, norb: int, nelec):
+    """Diagonal entries of a linear operator in the determinant basis
+    (python/ffsim/protocols/diagonal_protocol.py:40-50): ``obj._diag_``, else
+    TypeError."""
+    method = getattr(obj, "_diag_", None)
+    if method is not None:
+        return method(norb=norb, nelec=nelec)
+    raise TypeError(
+        f"Could not compute diagonal entries of object of type {type(obj)}.\n"
+        "The object did not have a _diag_ method that returned its diagonal entries."
+    )
+
+
+def trace(obj: Any, norb: int, nelec) -> float:
+    """Trace of a linear operator (python/ffsim/protocols/trace_protocol.py:34-47):
+    ``obj._trace_``, else the sum of ``obj._diag_``, else TypeError."""
+    method = getattr(obj, "_trace_", None)
+    if method is not None:
+        return method(norb=norb, nelec=nelec)
+    method = getattr(obj, "_diag_", None)
+    if method is not None:
+        return np.sum(method(norb=norb, nelec=nelec))
+    raise TypeError(
+        f"Could not compute trace of object of type {type(obj)}.\n"
+        "The object did not have a _trace_ method that returned the trace, or "
+        "a _diag_ method that returned its diagonal entries."
+    )

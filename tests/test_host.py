@@ -178,3 +178,22 @@ def test_approx_eq_protocol(rng):
     assert not ffsim_amd.approx_eq(op, same, rtol=0, atol=1e-12)
     assert not ffsim_amd.approx_eq(op, "x") and not ffsim_amd.approx_eq("x", op)
     assert ffsim_amd.approx_eq(3, 3) and not ffsim_amd.approx_eq(3, 4)
+
+
+def test_diag_and_trace_protocols(rng):
+    """diagonal_protocol.py:40-50, trace_protocol.py:34-47."""
+    from oracle import oracle as O
+
+    norb, nelec = 4, (2, 1)
+    h = ffsim_amd.random.random_diagonal_coulomb_hamiltonian(norb, real=True, seed=rng)
+    d = ffsim_amd.diag(h, norb, nelec)
+    np.testing.assert_allclose(d, h._diag_(norb, nelec))
+    dense = (h.constant * np.eye(len(d)) + O.one_body_dense(h.one_body_tensor, norb, nelec)
+             + np.diag(O.diag_coulomb_energies(
+                 (h.diag_coulomb_mats[0], h.diag_coulomb_mats[1], h.diag_coulomb_mats[0]),
+                 norb, nelec)))
+    np.testing.assert_allclose(ffsim_amd.trace(h, norb, nelec), np.trace(dense).real, atol=1e-10)
+    with pytest.raises(TypeError):
+        ffsim_amd.diag(object(), norb, nelec)
+    with pytest.raises(TypeError):
+        ffsim_amd.trace(object(), norb, nelec)

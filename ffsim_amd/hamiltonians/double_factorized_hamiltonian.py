@@ -84,6 +84,45 @@ class DoubleFactorizedHamiltonian:
         return dataclasses.replace(self, one_body_tensor=self.one_body_tensor - one_body,
                                    constant=self.constant - constant, z_representation=False)
 
+    def _diag_(self, norb: int, nelec) -> np.ndarray:
+        r"""Diagonal entries in the determinant basis (ffsim computes them
+        through the molecular Hamiltonian, double_factorized_hamiltonian.py:277-280).
+
+        Host numpy, by Wick's theorem on each determinant ``D``: with
+        :math:`\tilde n_i = \sum_{pq} U_{pi} U^*_{qi} a^\dagger_p a_q` and
+        :math:`M = U^\dagger P_D U` per spin, :math:`\langle \tilde n_i\rangle = M_{ii}`,
+        :math:`\langle \tilde n_i \tilde n_j\rangle = M_{ii} M_{jj} + \delta_{ij} M_{ii} - |M_{ij}|^2`
+        within a spin and :math:`M^\alpha_{ii} M^\beta_{jj}` across spins."""
+        from ffsim_amd.hamiltonians.diagonal_coulomb_hamiltonian import _occupation_matrix
+
+        assert isinstance(nelec, tuple)
+        occ = [_occupation_matrix(norb, n) for n in nelec]
+        kappa = np.diag(self.one_body_tensor).real
+        result = self.constant + (occ[0] @ kappa)[:, None] + (occ[1] @ kappa)[None, :]
+        for jmat, u in zip(np.asarray(self.diag_coulomb_mats, dtype=float),
+                           np.asarray(self.orbital_rotations)):
+            m, same = [], []
+            for o in occ:
+                mm = np.einsum("pi,ap,pj->aij", u.conj(), o, u)  # M per string
+                mi = np.einsum("aii->ai", mm).real
+                nn = mi[:, :, None] * mi[:, None, :] - np.abs(mm) ** 2
+                if self.z_representation:
+                    zz = 1 - 2 * mi[:, :, None] - 2 * mi[:, None, :] + 4 * nn
+                    off = jmat - np.diag(np.diag(jmat))
+                    same.append(0.125 * np.einsum("ij,aij->a", off, zz))
+                else:
+                    nn += np.einsum("ai,ij->aij", mi, np.eye(norb))
+                    same.append(0.5 * np.einsum("ij,aij->a", jmat, nn))
+                m.append(mi)
+            ma, mb = m
+            if self.z_representation:
+                za, zb = 1 - 2 * ma, 1 - 2 * mb
+                cross = 0.125 * (za @ jmat @ zb.T + (zb @ jmat @ za.T).T)
+            else:
+                cross = 0.5 * (ma @ jmat @ mb.T + (mb @ jmat @ ma.T).T)
+            result = result + same[0][:, None] + same[1][None, :] + cross
+        return np.asarray(result, dtype=float).reshape(-1)
+
     def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
         """double_factorized_hamiltonian.py:286-305."""
         if not isinstance(other, DoubleFactorizedHamiltonian):

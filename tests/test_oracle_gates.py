@@ -381,3 +381,20 @@ def test_df_z_representation_is_the_same_operator(norb, nelec, rng):
     assert not back.z_representation and ham.to_number_representation() is ham
     assert back._approx_eq_(ham, rtol=0, atol=1e-12)
     assert not z._approx_eq_(ham, rtol=1e-5, atol=1e-8)
+
+
+@pytest.mark.parametrize("z_rep", [False, True])
+@pytest.mark.parametrize("norb, nelec", [(4, (2, 1)), (5, (2, 2)), (3, (0, 3))])
+def test_df_hamiltonian_diag_vs_dense(norb, nelec, z_rep, rng):
+    """DoubleFactorizedHamiltonian._diag_ (double_factorized_hamiltonian.py:277-280)
+    equals the diagonal of the dense operator built from rotated eigenphase
+    energies."""
+    from ffsim_amd.hamiltonians import DoubleFactorizedHamiltonian
+
+    h = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    h = h + h.conj().T
+    mats = np.stack([O.random_symmetric_uniform(norb, rng) for _ in range(3)])
+    rots = np.stack([O.random_unitary(norb, rng) for _ in range(3)])
+    ham = DoubleFactorizedHamiltonian(h, mats, rots, constant=0.7, z_representation=z_rep)
+    dense = _dense_df(h, mats, rots, 0.7, norb, nelec, z_rep=z_rep)
+    np.testing.assert_allclose(ham._diag_(norb, nelec), np.diag(dense).real, atol=1e-11)

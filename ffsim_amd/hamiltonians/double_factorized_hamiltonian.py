@@ -10,6 +10,7 @@ import scipy.linalg
 import torch
 
 from ffsim_amd import _device, _lib
+from ffsim_amd import linalg as _linalg
 from ffsim_amd._linop import DeviceLinearOperator
 from ffsim_amd.contract.diag_coulomb import contract_diag_coulomb_into
 from ffsim_amd.contract.num_op_sum import contract_num_op_sum_into
@@ -49,6 +50,28 @@ class DoubleFactorizedHamiltonian:
     @property
     def norb(self) -> int:
         return self.one_body_tensor.shape[0]
+
+    @staticmethod
+    def from_molecular_hamiltonian(hamiltonian, *, z_representation: bool = False,
+                                   tol: float = 1e-8, max_vecs: int | None = None,
+                                   optimize: bool = False, cholesky: bool = True,
+                                   **optimize_kwargs) -> "DoubleFactorizedHamiltonian":
+        r"""Double-factorized form of a molecular Hamiltonian
+        (double_factorized_hamiltonian.py:140-222).  ``hamiltonian`` is any object
+        with ``one_body_tensor``, a real ``two_body_tensor`` in chemists' order
+        :math:`(pq|rs)` and ``constant`` (ffsim's MolecularHamiltonian is one).
+        The one-body tensor becomes :math:`h_{pq} - \tfrac12 \sum_r (pr|rq)`
+        and the two-body tensor is factored by
+        :func:`ffsim_amd.linalg.double_factorized`."""
+        two_body = np.asarray(hamiltonian.two_body_tensor)
+        one_body = np.asarray(hamiltonian.one_body_tensor) - 0.5 * np.einsum("prrq", two_body)
+        mats, rots = _linalg.double_factorized(two_body, tol=tol, max_vecs=max_vecs,
+                                               optimize=optimize, cholesky=cholesky,
+                                               **optimize_kwargs)
+        ham = DoubleFactorizedHamiltonian(one_body_tensor=one_body, diag_coulomb_mats=mats,
+                                          orbital_rotations=rots,
+                                          constant=float(hamiltonian.constant))
+        return ham.to_z_representation() if z_representation else ham
 
     def _z_correction(self):
         r"""One-body and constant terms that move the operator between the

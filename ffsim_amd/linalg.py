@@ -238,3 +238,80 @@ def double_factorized_t2_alpha_beta(t2_amplitudes: np.ndarray, *, tol: float = 1
                      big[:, :, norb:, norb:]], axis=2)
     return (mats.reshape(-1, 3, norb, norb)[:max_terms],
             rotations.reshape(-1, 2, norb, norb)[:max_terms])
+
+
+# ---------------------------------------------------------------------------
+# Double factorization of a two-body tensor (explicit forms).
+
+
+def modified_cholesky(mat: np.ndarray, *, tol: float = 1e-8, max_vecs: int | None = None):
+    r"""Pivoted (modified) Cholesky vectors of a positive semidefinite matrix,
+    :math:`M \approx \sum_i v_i v_i^\dagger`
+    (python/ffsim/linalg/double_factorized_decomposition.py:70-131; Motta and
+    Zhang, arXiv:1711.02242).  Each step takes the column of the largest
+    remaining diagonal error, stopping once it falls below ``tol`` or after
+    ``max_vecs`` vectors.  Returns the vectors as the columns of a
+    ``(dim, n_vecs)`` array."""
+    mat = np.asarray(mat)
+    dim = mat.shape[0]
+    if not dim:
+        return np.empty((0, 0))
+    if max_vecs is None:
+        max_vecs = dim
+    vecs = np.zeros((dim, max_vecs + 1), dtype=mat.dtype)
+    err = np.diagonal(mat).real.copy()
+    n = 0
+    while n <= max_vecs:
+        piv = int(np.argmax(err))
+        if err[piv] < tol:
+            break
+        v = mat[:, piv].copy()
+        if n:
+            v -= vecs[:, :n] @ vecs[piv, :n].conj()
+        v /= np.sqrt(err[piv])
+        vecs[:, n] = v
+        err -= np.abs(v) ** 2
+        n += 1
+    return vecs[:, : min(n, max_vecs)]
+
+
+def double_factorized(two_body_tensor: np.ndarray, *, tol: float = 1e-8,
+                      max_vecs: int | None = None, optimize: bool = False,
+                      cholesky: bool = True, **optimize_kwargs):
+    r"""Double factorization of a real two-body tensor,
+    :math:`h_{pqrs} = \sum_t \sum_{ij} Z^{(t)}_{ij} U^{(t)}_{pi} U^{(t)*}_{qi}
+    U^{(t)}_{rj} U^{(t)*}_{sj}` (python/ffsim/linalg/double_factorized_decomposition.py:145-311).
+
+    The ``(pq), (rs)`` matrix of the tensor is factored into vectors (pivoted
+    Cholesky, or with ``cholesky=False`` its eigendecomposition, largest
+    |eigenvalue| first); every vector, read as an ``norb x norb`` symmetric
+    matrix, is diagonalized, ``U`` being its eigenvectors and ``Z`` the outer
+    product of its eigenvalues (times the eigenvalue of the outer
+    decomposition in the eigh form).  Returns ``(diag_coulomb_mats,
+    orbital_rotations)``, each ``(L, norb, norb)``.  ``optimize=True`` raises
+    ``NotImplementedError``; the keyword arguments only the optimized form
+    reads are accepted and ignored."""
+    check_optimize_kwargs(optimize_kwargs, OPTIMIZE_ARGS | {"diag_coulomb_indices",
+                                                            "return_optimize_result"})
+    two_body_tensor = np.asarray(two_body_tensor)
+    if np.iscomplexobj(two_body_tensor):
+        raise ValueError("Double factorization of complex two-body tensors is not supported.")
+    norb = two_body_tensor.shape[0]
+    if not norb:
+        return np.empty((0, 0, 0)), np.empty((0, 0, 0))
+    if optimize:
+        raise NotImplementedError(
+            "optimize=True (compressed double factorization) is not supported; "
+            "only the explicit factorization is available.")
+    if max_vecs is None:
+        max_vecs = norb * (norb + 1) // 2
+    flat = two_body_tensor.reshape(norb * norb, norb * norb)
+    if cholesky:
+        vecs = modified_cholesky(flat, tol=tol, max_vecs=max_vecs)
+        weights = np.ones(vecs.shape[1])
+    else:
+        weights, vecs = _ranked_eigh(flat, tol)
+        weights, vecs = weights[:max_vecs], vecs[:, :max_vecs]
+    e, rotations = np.linalg.eigh(vecs.T.reshape(-1, norb, norb))
+    mats = weights[:, None, None] * e[:, :, None] * e[:, None, :]
+    return mats, rotations

@@ -48,7 +48,7 @@ def test_public_api_names():
                  "num_op_sum_linop", "contract_one_body", "one_body_linop"]:
         assert hasattr(ffsim_amd.contract, name), name
     for name in ["givens_decomposition", "double_factorized_t2",
-                 "double_factorized_t2_alpha_beta"]:
+                 "double_factorized_t2_alpha_beta", "double_factorized", "modified_cholesky"]:
         assert hasattr(ffsim_amd.linalg, name), name
     for name in ["random_t2_amplitudes", "random_unitary", "random_ucj_op_spin_balanced"]:
         assert hasattr(ffsim_amd.random, name), name
@@ -56,6 +56,7 @@ def test_public_api_names():
                 ffsim_amd.UCJOpSpinUnbalanced, ffsim_amd.UCJAnglesOpSpinBalanced]:
         assert hasattr(cls, "from_t_amplitudes"), cls
     assert hasattr(ffsim_amd.UCJOpSpinBalanced, "from_cisd_vec")
+    assert hasattr(ffsim_amd.DoubleFactorizedHamiltonian, "from_molecular_hamiltonian")
 
 
 def test_apply_unitary_type_error():

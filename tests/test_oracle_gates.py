@@ -10,6 +10,7 @@ restatement is caught by these pins.
 
 from __future__ import annotations
 
+import itertools
 import math
 from pathlib import Path
 
@@ -398,3 +399,81 @@ def test_df_hamiltonian_diag_vs_dense(norb, nelec, z_rep, rng):
     ham = DoubleFactorizedHamiltonian(h, mats, rots, constant=0.7, z_representation=z_rep)
     dense = _dense_df(h, mats, rots, 0.7, norb, nelec, z_rep=z_rep)
     np.testing.assert_allclose(ham._diag_(norb, nelec), np.diag(dense).real, atol=1e-11)
+
+
+def _dense_molecular(h1, h2, constant, norb, nelec):
+    """Dense sum h1_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - delta_qr E_ps) + constant,
+    with E_pq the spin-summed one-body excitation built from creation and
+    annihilation signs."""
+    def e(p, q):
+        m = np.zeros((norb, norb))
+        m[p, q] = 1
+        return O.one_body_dense(m, norb, nelec)
+
+    E = [[e(p, q) for q in range(norb)] for p in range(norb)]
+    dim = E[0][0].shape[0]
+    H = constant * np.eye(dim, dtype=complex)
+    for p, q in itertools.product(range(norb), repeat=2):
+        H += h1[p, q] * E[p][q]
+        for r, s in itertools.product(range(norb), repeat=2):
+            if h2[p, q, r, s]:
+                H += 0.5 * h2[p, q, r, s] * (E[p][q] @ E[r][s] - (q == r) * E[p][s])
+    return H
+
+
+def _random_two_body(norb, rng, rank=5):
+    """Real (pq|rs) with the 8-fold symmetry and a positive semidefinite
+    (pq),(rs) matrix: sum_k L_k (x) L_k with symmetric L_k."""
+    L = rng.standard_normal((rank, norb, norb))
+    L = L + L.transpose(0, 2, 1)
+    return np.einsum("kpq,krs->pqrs", L, L)
+
+
+@pytest.mark.parametrize("cholesky", [True, False])
+@pytest.mark.parametrize("z_rep", [False, True])
+def test_df_from_molecular_hamiltonian_vs_dense(cholesky, z_rep, rng):
+    """DoubleFactorizedHamiltonian.from_molecular_hamiltonian
+    (double_factorized_hamiltonian.py:140-222) with linalg.double_factorized
+    (double_factorized_decomposition.py:145-311): the dense double-factorized
+    operator equals the dense molecular Hamiltonian; the factors rebuild the
+    two-body tensor."""
+    from types import SimpleNamespace
+
+    from ffsim_amd import linalg
+    from ffsim_amd.hamiltonians import DoubleFactorizedHamiltonian
+
+    norb, nelec = 4, (2, 1)
+    h1 = rng.standard_normal((norb, norb))
+    h1 = h1 + h1.T
+    h2 = _random_two_body(norb, rng)
+    mol = SimpleNamespace(one_body_tensor=h1, two_body_tensor=h2, constant=0.3)
+    df = DoubleFactorizedHamiltonian.from_molecular_hamiltonian(mol, cholesky=cholesky,
+                                                                z_representation=z_rep)
+    assert df.z_representation == z_rep
+    dense = _dense_df(df.one_body_tensor, df.diag_coulomb_mats, df.orbital_rotations,
+                      df.constant, norb, nelec, z_rep=z_rep)
+    np.testing.assert_allclose(dense, _dense_molecular(h1, h2, 0.3, norb, nelec), atol=1e-10)
+    mats, rots = linalg.double_factorized(h2, cholesky=cholesky)
+    rebuilt = np.einsum("kij,kpi,kqi,krj,ksj->pqrs", mats, rots, rots.conj(), rots, rots.conj())
+    np.testing.assert_allclose(rebuilt, h2, atol=1e-10)
+
+
+def test_modified_cholesky_and_truncation(rng):
+    """modified_cholesky (double_factorized_decomposition.py:70-131): exact for a
+    positive semidefinite matrix, truncated by max_vecs, stopped by tol."""
+    from ffsim_amd import linalg
+
+    a = rng.standard_normal((8, 5)) + 1j * rng.standard_normal((8, 5))
+    m = a @ a.conj().T  # rank 5
+    v = linalg.modified_cholesky(m)
+    assert v.shape == (8, 5)
+    np.testing.assert_allclose(v @ v.conj().T, m, atol=1e-10)
+    assert linalg.modified_cholesky(m, max_vecs=3).shape == (8, 3)
+    assert linalg.modified_cholesky(np.zeros((3, 3))).shape == (3, 0)
+    h2 = _random_two_body(4, rng, rank=6)
+    mats, _ = linalg.double_factorized(h2, max_vecs=2)
+    assert len(mats) == 2
+    with pytest.raises(ValueError):
+        linalg.double_factorized(h2 + 0j)
+    with pytest.raises(NotImplementedError):
+        linalg.double_factorized(h2, optimize=True)

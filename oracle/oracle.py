@@ -894,6 +894,27 @@ def one_body_dense(h: np.ndarray, norb: int, nelec) -> np.ndarray:
     return np.kron(Ha, np.eye(len(sb))) + np.kron(np.eye(len(sa)), Hb)
 
 
+def molecular_hamiltonian_dense(h1, h2, constant, norb, nelec) -> np.ndarray:
+    """Dense sum_pq h1_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - delta_qr E_ps) +
+    constant on the (na, nb) sector, with E_pq the spin-summed one-body
+    excitation from one_body_dense (chemists' order for h2)."""
+    def e(p, q):
+        m = np.zeros((norb, norb))
+        m[p, q] = 1
+        return one_body_dense(m, norb, nelec)
+
+    E = [[e(p, q) for q in range(norb)] for p in range(norb)]
+    H = constant * np.eye(E[0][0].shape[0], dtype=complex)
+    for p in range(norb):
+        for q in range(norb):
+            H += h1[p, q] * E[p][q]
+            for r in range(norb):
+                for s in range(norb):
+                    if h2[p, q, r, s]:
+                        H += 0.5 * h2[p, q, r, s] * (E[p][q] @ E[r][s] - (q == r) * E[p][s])
+    return H
+
+
 def diag_coulomb_energies(mat, norb, nelec, z_representation=False) -> np.ndarray:
     """E(a, b) on every determinant from occupation numbers (closed form).
 

@@ -10,7 +10,6 @@ restatement is caught by these pins.
 
 from __future__ import annotations
 
-import itertools
 import math
 from pathlib import Path
 
@@ -401,26 +400,6 @@ def test_df_hamiltonian_diag_vs_dense(norb, nelec, z_rep, rng):
     np.testing.assert_allclose(ham._diag_(norb, nelec), np.diag(dense).real, atol=1e-11)
 
 
-def _dense_molecular(h1, h2, constant, norb, nelec):
-    """Dense sum h1_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - delta_qr E_ps) + constant,
-    with E_pq the spin-summed one-body excitation built from creation and
-    annihilation signs."""
-    def e(p, q):
-        m = np.zeros((norb, norb))
-        m[p, q] = 1
-        return O.one_body_dense(m, norb, nelec)
-
-    E = [[e(p, q) for q in range(norb)] for p in range(norb)]
-    dim = E[0][0].shape[0]
-    H = constant * np.eye(dim, dtype=complex)
-    for p, q in itertools.product(range(norb), repeat=2):
-        H += h1[p, q] * E[p][q]
-        for r, s in itertools.product(range(norb), repeat=2):
-            if h2[p, q, r, s]:
-                H += 0.5 * h2[p, q, r, s] * (E[p][q] @ E[r][s] - (q == r) * E[p][s])
-    return H
-
-
 def _random_two_body(norb, rng, rank=5):
     """Real (pq|rs) with the 8-fold symmetry and a positive semidefinite
     (pq),(rs) matrix: sum_k L_k (x) L_k with symmetric L_k."""
@@ -452,7 +431,8 @@ def test_df_from_molecular_hamiltonian_vs_dense(cholesky, z_rep, rng):
     assert df.z_representation == z_rep
     dense = _dense_df(df.one_body_tensor, df.diag_coulomb_mats, df.orbital_rotations,
                       df.constant, norb, nelec, z_rep=z_rep)
-    np.testing.assert_allclose(dense, _dense_molecular(h1, h2, 0.3, norb, nelec), atol=1e-10)
+    np.testing.assert_allclose(dense, O.molecular_hamiltonian_dense(h1, h2, 0.3, norb, nelec),
+                               atol=1e-10)
     mats, rots = linalg.double_factorized(h2, cholesky=cholesky)
     rebuilt = np.einsum("kij,kpi,kqi,krj,ksj->pqrs", mats, rots, rots.conj(), rots, rots.conj())
     np.testing.assert_allclose(rebuilt, h2, atol=1e-10)

@@ -14,7 +14,14 @@ from ffsim_amd.gates import (
 )
 from ffsim_amd.hamiltonians import DiagonalCoulombHamiltonian, DoubleFactorizedHamiltonian
 from ffsim_amd.protocols import apply_unitary, approx_eq, diag, linear_operator, trace
-from ffsim_amd.states import dim, dims, hartree_fock_state
+from ffsim_amd.states import (
+    BitstringType,
+    addresses_to_strings,
+    dim,
+    dims,
+    hartree_fock_state,
+    strings_to_addresses,
+)
 from ffsim_amd.trotter import (
     simulate_trotter_diag_coulomb_split_op,
     simulate_trotter_double_factorized,
@@ -29,6 +36,9 @@ from ffsim_amd.variational import (
 )
 
 __all__ = [
+    "BitstringType",
+    "addresses_to_strings",
+    "strings_to_addresses",
     "DiagonalCoulombHamiltonian",
     "DoubleFactorizedHamiltonian",
     "GivensAnsatzOp",

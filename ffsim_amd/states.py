@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import math
+from enum import Enum, auto
 
 import numpy as np
 
@@ -27,3 +28,144 @@ def hartree_fock_state(norb: int, nelec) -> np.ndarray:
     vec = np.zeros(dim(norb, nelec), dtype=complex)
     vec[0] = 1
     return vec
+
+
+# ---------------------------------------------------------------------------
+# Address <-> bitstring conversion (python/ffsim/states/bitstring.py).  The
+# address of a string with occupied orbitals c_1 < ... < c_k is its colex rank
+# sum_t C(c_t, t); a spinful address is a * dim_b + b and its bitstring puts
+# the beta string above the alpha one, (beta << norb) | alpha.
+
+
+class BitstringType(Enum):
+    """Bitstring formats (bitstring.py:21-45): STRING ("0101", most significant
+    bit first), INT (5) and BIT_ARRAY (a boolean row, most significant first)."""
+
+    STRING = auto()
+    INT = auto()
+    BIT_ARRAY = auto()
+
+
+def convert_bitstring_type(strings, input_type: BitstringType, output_type: BitstringType,
+                           length: int):
+    """Convert bitstrings between formats (bitstring.py:48-114)."""
+    for t, name in ((input_type, "input_type"), (output_type, "output_type")):
+        if not isinstance(t, BitstringType):
+            raise TypeError(
+                f"{name} must be ffsim.BitstringType.STRING, ffsim.BitstringType.INT, "
+                f"or ffsim.BitstringType.BIT_ARRAY. Got {t}.")
+    if input_type is output_type:
+        return strings
+    if input_type is BitstringType.BIT_ARRAY:
+        ints = [sum(1 << i for i, bit in enumerate(row[::-1]) if bit) for row in strings]
+    elif input_type is BitstringType.STRING:
+        ints = [int(s, 2) for s in strings]
+    else:
+        ints = [int(s) for s in strings]
+    if output_type is BitstringType.INT:
+        return ints
+    if output_type is BitstringType.STRING:
+        return [format(s, f"0{length}b") for s in ints]
+    if not ints:
+        return np.empty((0, length), dtype=bool)
+    return np.array([[(s >> i) & 1 for i in range(length - 1, -1, -1)] for s in ints],
+                    dtype=bool)
+
+
+def concatenate_bitstrings(strings_a, strings_b, bitstring_type: BitstringType, length: int):
+    """Beta strings above alpha strings (bitstring.py:145-170)."""
+    if bitstring_type is BitstringType.STRING:
+        return [b + a for a, b in zip(strings_a, strings_b)]
+    if bitstring_type is BitstringType.INT:
+        return [(b << length) | a for a, b in zip(strings_a, strings_b)]
+    return np.concatenate([strings_b, strings_a], axis=1)
+
+
+def restrict_bitstrings(strings, indices, bitstring_type: BitstringType):
+    """Keep the bits at ``indices`` (bitstring.py:117-142)."""
+    if bitstring_type is BitstringType.STRING:
+        return ["".join(s[-1 - i] for i in indices[::-1]) for s in strings]
+    if bitstring_type is BitstringType.INT:
+        return [sum(((s >> idx) & 1) << i for i, idx in enumerate(indices)) for s in strings]
+    return np.asarray(strings)[:, [-1 - i for i in indices[::-1]]]
+
+
+def _colex_unrank(addresses, norb: int, nocc: int) -> list[int]:
+    """Strings of the given colex ranks: for t = nocc..1 the largest c with
+    C(c, t) <= the remaining rank is the t-th occupied orbital."""
+    out = []
+    for addr in addresses:
+        s, r = 0, int(addr)
+        if not 0 <= r < math.comb(norb, nocc):
+            raise ValueError(f"address {addr} out of range for norb={norb}, nocc={nocc}")
+        c = norb
+        for t in range(nocc, 0, -1):
+            c -= 1
+            while math.comb(c, t) > r:
+                c -= 1
+            s |= 1 << c
+            r -= math.comb(c, t)
+        out.append(s)
+    return out
+
+
+def _colex_rank(strings, norb: int, nocc: int) -> np.ndarray:
+    out = np.zeros(len(strings), dtype=np.int64)
+    for k, s in enumerate(strings):
+        s = int(s)
+        if s >> norb or bin(s).count("1") != nocc:
+            raise ValueError(f"bitstring {s:b} is not a string of {nocc} in {norb} orbitals")
+        rank, t = 0, 0
+        while s:
+            c = (s & -s).bit_length() - 1
+            t += 1
+            rank += math.comb(c, t)
+            s &= s - 1
+        out[k] = rank
+    return out
+
+
+def addresses_to_strings(addresses, norb: int, nelec, concatenate: bool = True,
+                         bitstring_type: BitstringType = BitstringType.INT):
+    """Bitstrings of state-vector addresses (bitstring.py:173-263).  Spinful:
+    concatenated (beta above alpha) or, with ``concatenate=False``, the pair
+    (alpha strings, beta strings)."""
+    addresses = np.asarray(addresses, dtype=np.int64).reshape(-1)
+    if isinstance(nelec, int):
+        return convert_bitstring_type(_colex_unrank(addresses, norb, nelec), BitstringType.INT,
+                                      bitstring_type, norb)
+    n_alpha, n_beta = nelec
+    addr_a, addr_b = np.divmod(addresses, math.comb(norb, n_beta))
+    strings_a = convert_bitstring_type(_colex_unrank(addr_a, norb, n_alpha), BitstringType.INT,
+                                       bitstring_type, norb)
+    strings_b = convert_bitstring_type(_colex_unrank(addr_b, norb, n_beta), BitstringType.INT,
+                                       bitstring_type, norb)
+    if concatenate:
+        return concatenate_bitstrings(strings_a, strings_b, bitstring_type, norb)
+    return strings_a, strings_b
+
+
+def strings_to_addresses(strings, norb: int, nelec) -> np.ndarray:
+    """State-vector addresses of bitstrings (bitstring.py:266-361); the
+    format is read from the input (2-D array: bits; str: strings; else ints)."""
+    if not len(strings):
+        return np.array([])
+    if isinstance(strings, np.ndarray) and strings.ndim == 2:
+        kind = BitstringType.BIT_ARRAY
+    elif isinstance(strings[0], str):
+        kind = BitstringType.STRING
+    else:
+        kind = BitstringType.INT
+    ints = convert_bitstring_type(strings, kind, BitstringType.INT, norb)
+    if isinstance(nelec, int):
+        return _colex_rank(ints, norb, nelec)
+    n_alpha, n_beta = nelec
+    mask = (1 << norb) - 1
+    addr_a = _colex_rank([s & mask for s in ints], norb, n_alpha)
+    addr_b = _colex_rank([s >> norb for s in ints], norb, n_beta)
+    return addr_a * math.comb(norb, n_beta) + addr_b
+
+
+def bitstring_to_occupied_orbitals(bitstring: int) -> list[int]:
+    """Positions of the set bits (bitstring.py:364-366)."""
+    return [i for i in range(int(bitstring).bit_length()) if (int(bitstring) >> i) & 1]

@@ -90,39 +90,67 @@ def restrict_bitstrings(strings, indices, bitstring_type: BitstringType):
     return np.asarray(strings)[:, [-1 - i for i in indices[::-1]]]
 
 
+def _binom_table(norb: int) -> np.ndarray:
+    """C(c, t) for 0 <= c < norb and 0 <= t <= norb + 1, as int64 (norb <= 62)."""
+    table = np.zeros((max(norb, 1), norb + 2), dtype=np.int64)
+    for c in range(norb):
+        for t in range(c + 1):
+            table[c, t] = math.comb(c, t)
+    return table
+
+
 def _colex_unrank(addresses, norb: int, nocc: int) -> list[int]:
     """Strings of the given colex ranks: for t = nocc..1 the largest c with
     C(c, t) <= the remaining rank is the t-th occupied orbital."""
-    out = []
-    for addr in addresses:
-        s, r = 0, int(addr)
-        if not 0 <= r < math.comb(norb, nocc):
-            raise ValueError(f"address {addr} out of range for norb={norb}, nocc={nocc}")
-        c = norb
-        for t in range(nocc, 0, -1):
-            c -= 1
-            while math.comb(c, t) > r:
+    addr = np.asarray(addresses, dtype=np.int64).reshape(-1)
+    if addr.size and (addr.min() < 0 or addr.max() >= math.comb(norb, nocc)):
+        raise ValueError(f"address out of range for norb={norb}, nocc={nocc}")
+    if norb > 62:
+        out = []
+        for a in addr:
+            s, r, c = 0, int(a), norb
+            for t in range(nocc, 0, -1):
                 c -= 1
-            s |= 1 << c
-            r -= math.comb(c, t)
-        out.append(s)
-    return out
+                while math.comb(c, t) > r:
+                    c -= 1
+                s |= 1 << c
+                r -= math.comb(c, t)
+            out.append(s)
+        return out
+    table = _binom_table(norb)
+    s = np.zeros(addr.shape, dtype=np.int64)
+    r = addr.copy()
+    for t in range(nocc, 0, -1):
+        c = np.searchsorted(table[:, t], r, side="right") - 1
+        s |= np.left_shift(np.int64(1), c)
+        r -= table[c, t]
+    return [int(x) for x in s]
 
 
 def _colex_rank(strings, norb: int, nocc: int) -> np.ndarray:
-    out = np.zeros(len(strings), dtype=np.int64)
-    for k, s in enumerate(strings):
-        s = int(s)
-        if s >> norb or bin(s).count("1") != nocc:
-            raise ValueError(f"bitstring {s:b} is not a string of {nocc} in {norb} orbitals")
-        rank, t = 0, 0
-        while s:
-            c = (s & -s).bit_length() - 1
-            t += 1
-            rank += math.comb(c, t)
-            s &= s - 1
-        out[k] = rank
-    return out
+    """Colex ranks sum_t C(c_t, t) of strings with nocc of norb bits set."""
+    ints = [int(x) for x in strings]
+    if any(x < 0 or x >> norb or bin(x).count("1") != nocc for x in ints):
+        raise ValueError(f"a bitstring is not a string of {nocc} in {norb} orbitals")
+    if norb > 62:
+        out = np.zeros(len(ints), dtype=np.int64)
+        for k, x in enumerate(ints):
+            rank, t = 0, 0
+            while x:
+                t += 1
+                rank += math.comb((x & -x).bit_length() - 1, t)
+                x &= x - 1
+            out[k] = rank
+        return out
+    table = _binom_table(norb)
+    s = np.asarray(ints, dtype=np.int64)
+    rank = np.zeros(s.shape, dtype=np.int64)
+    t = np.zeros(s.shape, dtype=np.int64)
+    for c in range(norb):
+        bit = (s >> c) & 1
+        t += bit
+        rank += bit * table[c, t]
+    return rank
 
 
 def addresses_to_strings(addresses, norb: int, nelec, concatenate: bool = True,

@@ -68,3 +68,15 @@ def test_formats_round_trip():
         S.convert_bitstring_type(ints, "int", B.STRING, 4)
     with pytest.raises(ValueError):
         S.strings_to_addresses([0b111], 3, 2)
+
+
+def test_wide_strings_round_trip():
+    """More than 62 orbitals take the Python-integer path; both paths agree
+    where they overlap."""
+    for norb, nocc in ((64, 2), (70, 3), (40, 3)):
+        n = math.comb(norb, nocc)
+        addrs = np.unique(np.linspace(0, n - 1, 97).astype(np.int64))
+        strings = S.addresses_to_strings(addrs, norb, nocc)
+        assert all(bin(x).count("1") == nocc and x >> norb == 0 for x in strings)
+        assert strings == sorted(strings)
+        np.testing.assert_array_equal(S.strings_to_addresses(strings, norb, nocc), addrs)

@@ -44,6 +44,9 @@ SIGNATURES: dict[str, list] = {
     "ffsim_diag_coulomb_grad_reduce": [_P, _P, _I, _I, _I, _I, _P, _P],
     "ffsim_transition_rdm": [_P, _P, _I, _I, _I, _I, _P, _P],
     "ffsim_apply_diag_coulomb_sequence": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _I, _P],
+    "ffsim_apply_givens_rotation_in_place": [_P, _I, _I64, _I64, _D, _D, _D, _P, _P, _I64, _P],
+    "ffsim_apply_phase_shift_in_place": [_P, _I, _I64, _I64, _D, _D, _P, _I64, _P],
+    "ffsim_apply_givens_layer": [_P, _I, _I, _I, _I64, _I, _I, _P, _P, _P, _P, _P],
 }
 
 

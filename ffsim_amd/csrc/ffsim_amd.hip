@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -164,6 +165,13 @@ struct DevWindow {
   int n_fibers = 0, n_active = 0, max_rows = 0, max_rows_active = 0, lo = 0, w = 0;
   long rows_all = 0, rows_active = 0;  // strings covered by all / by the active fibers
 };
+// Pair tables of one rotation (i, j) for (norb, nocc): the strings with i
+// occupied / j empty in ascending address order, and their partners.
+struct DevPairs {
+  int64_t* first = nullptr;
+  int64_t* second = nullptr;
+  long n = 0;
+};
 struct DevSchedule {
   uint32_t* chunks = nullptr;
   int32_t* index = nullptr;
@@ -182,18 +190,24 @@ struct DevGroups {
 };
 
 // Program ring: per-call gate parameters (Givens steps, phases, diagonal
-// operator coefficients) are written to a pinned host slot, copied once per
-// call to the matching device slot, and the slot is recycled only after the
-// event recorded behind the call's last kernel has completed.
+// operator coefficients) are staged on the host, copied once per call through
+// a pinned slot to the matching device slot, and a slot is reused only after
+// the event recorded behind the call's last kernel has completed.  A slot's
+// device buffer holds the parameters followed by the call's device-only
+// scratch (written by kernels, never copied).  Slot buffers are allocated on
+// first use and grown when a call needs more.
 struct ProgramRing {
-  static constexpr int kSlots = 16;
-  static constexpr size_t kSlotBytes = 8u << 20;      // host-written parameters (pinned)
-  static constexpr size_t kDevSlotBytes = 32u << 20;  // device slot: parameters + scratch
+  static constexpr int kSlots = 64;
+  static constexpr size_t kMinBytes = 1u << 20;
+  static constexpr size_t kMaxBytes = 256u << 20;  // parameters + scratch of one call
   char* h[kSlots] = {};
   char* d[kSlots] = {};
+  size_t cap_h[kSlots] = {}, cap_d[kSlots] = {};
   hipEvent_t ev[kSlots] = {};
-  bool armed[kSlots] = {};
+  bool armed[kSlots] = {};  // ev[slot] is recorded behind the slot's last use
+  bool busy[kSlots] = {};   // a call owns the slot (from Program construction to finish)
   int next = 0;
+  std::condition_variable freed;
 };
 
 struct Context {
@@ -205,6 +219,7 @@ struct Context {
   std::map<std::pair<int, int>, DevLinks> links;
   std::map<std::tuple<int, int, int, int>, DevWindow> windows;
   std::map<std::pair<int, bool>, DevGroups> groups;  // (window width, merged complements)
+  std::map<std::tuple<int, int, int, int>, DevPairs> pairs;  // (norb, nocc, i, j)
   ProgramRing ring;
 };
 
@@ -264,11 +279,8 @@ Context& context() {
   ctx->lds_max = prop.sharedMemPerBlock;
   ctx->n_cu = prop.multiProcessorCount;
   g_n_cu = std::max(1, prop.multiProcessorCount);
-  for (int k = 0; k < ProgramRing::kSlots; ++k) {
-    HIP_CHECK(hipHostMalloc((void**)&ctx->ring.h[k], ProgramRing::kSlotBytes, hipHostMallocDefault));
-    HIP_CHECK(hipMalloc((void**)&ctx->ring.d[k], ProgramRing::kDevSlotBytes));
+  for (int k = 0; k < ProgramRing::kSlots; ++k)
     HIP_CHECK(hipEventCreateWithFlags(&ctx->ring.ev[k], hipEventDisableTiming));
-  }
   Context& ref = *ctx;
   g_ctx[dev] = std::move(ctx);
   return ref;
@@ -353,6 +365,27 @@ const DevWindow& dev_window(Context& ctx, int norb, int nocc, int lo, int w) {
   return ctx.windows[key] = dw;
 }
 
+const DevPairs& dev_pairs(Context& ctx, int norb, int nocc, int i, int j) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  auto key = std::make_tuple(norb, nocc, i, j);
+  auto it = ctx.pairs.find(key);
+  if (it != ctx.pairs.end()) return it->second;
+  std::vector<uint64_t> s = make_strings(norb, nocc);
+  std::vector<int64_t> first, second;
+  for (size_t a = 0; a < s.size(); ++a) {
+    const uint64_t b = s[a];
+    if (((b >> i) & 1) && !((b >> j) & 1)) {
+      first.push_back((int64_t)a);
+      second.push_back((int64_t)string_rank(b ^ (1ull << i) ^ (1ull << j)));
+    }
+  }
+  DevPairs dp;
+  dp.n = (long)first.size();
+  dp.first = upload_table(first);
+  dp.second = upload_table(second);
+  return ctx.pairs[key] = dp;
+}
+
 // Group tables of a window width.  c64 tables merge complemented groups into
 // the mirrored class (fewer, fuller chunks: 4.6% faster); the c128 kernel keeps
 // Re(sigma) in SGPRs, which complemented lanes cannot share (4% slower merged).
@@ -387,38 +420,82 @@ const DevSchedule& dev_schedule(Context& ctx, const DevGroups& gr, int n_waves, 
   return g.schedules[key] = ds;
 }
 
-// A call's program: host bytes appended, device address known after commit.
+size_t pow2_ceil(size_t x) {
+  size_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// A call's program: parameter bytes are appended on the host, device-only
+// scratch is reserved, and both get device addresses at commit().  The slot is
+// the call's from construction until finish() (or destruction): other threads
+// skip it, and waiting for its previous use happens outside the context lock.
 struct Program {
   Context* ctx;
-  int slot;
-  size_t used = 0;
+  int slot = -1;
   hipStream_t stream;
-  bool committed = false;
+  std::vector<char> host;      // parameter bytes
+  size_t scratch_bytes = 0;    // device-only scratch, placed after the parameters
+  size_t scratch_base = 0;     // device offset of the scratch (known at commit)
+  bool committed = false, finished = false;
 
   Program(Context& c, hipStream_t s) : ctx(&c), stream(s) {
-    std::lock_guard<std::mutex> lk(c.mu);
-    slot = c.ring.next;
-    c.ring.next = (c.ring.next + 1) % ProgramRing::kSlots;
-    if (c.ring.armed[slot]) HIP_CHECK(hipEventSynchronize(c.ring.ev[slot]));
-    c.ring.armed[slot] = false;
+    ProgramRing& R = c.ring;
+    std::unique_lock<std::mutex> lk(c.mu);
+    for (;;) {
+      for (int k = 0; k < ProgramRing::kSlots && slot < 0; ++k) {
+        const int cand = (R.next + k) % ProgramRing::kSlots;
+        if (!R.busy[cand]) slot = cand;
+      }
+      if (slot >= 0) break;
+      R.freed.wait(lk);
+    }
+    R.next = (slot + 1) % ProgramRing::kSlots;
+    R.busy[slot] = true;
+    const bool wait = R.armed[slot];
+    R.armed[slot] = false;
+    lk.unlock();
+    if (wait) {
+      const hipError_t e = hipEventSynchronize(R.ev[slot]);
+      if (e != hipSuccess) {
+        release(false);
+        throw Failure{FFSIM_ERR_HIP, std::string("program slot wait: ") + hipGetErrorString(e)};
+      }
+    }
   }
-  size_t scratch_lo = ProgramRing::kDevSlotBytes;  // device-only scratch grows down from the top
-  // Appends n bytes (16-byte aligned), returns the offset.
+  Program(const Program&) = delete;
+  Program& operator=(const Program&) = delete;
+  ~Program() {
+    if (finished) return;
+    // an error left the call unfinished: anything already enqueued keeps the
+    // slot until it has run
+    bool armed = false;
+    if (committed) armed = hipEventRecord(ctx->ring.ev[slot], stream) == hipSuccess;
+    release(armed);
+  }
+  void release(bool armed) {
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->ring.armed[slot] = armed;
+      ctx->ring.busy[slot] = false;
+    }
+    ctx->ring.freed.notify_one();
+  }
+  // Appends n bytes (16-byte aligned; zeros when src is null), returns the offset.
   size_t append(const void* src, size_t n) {
-    size_t off = (used + 15) & ~size_t(15);
-    require(off + n <= std::min(scratch_lo, ProgramRing::kSlotBytes), FFSIM_ERR_VALUE,
+    const size_t off = (host.size() + 15) & ~size_t(15);
+    require(off + n + scratch_bytes <= ProgramRing::kMaxBytes, FFSIM_ERR_VALUE,
             "gate program too large");
-    if (src) std::memcpy(ctx->ring.h[slot] + off, src, n);
-    used = off + n;
+    host.resize(off + n, 0);
+    if (src && n) std::memcpy(host.data() + off, src, n);
     return off;
   }
-  // Reserves n bytes of device-only scratch (written by kernels, never copied
-  // from the host); false when the slot has no room left.
+  // Reserves n bytes of device-only scratch; false when the call would exceed
+  // the slot limit.  Addresses via scratch<T>(off) after commit().
   bool reserve(size_t n, size_t* off) {
-    if (n > scratch_lo) return false;
-    const size_t o = (scratch_lo - n) & ~size_t(15);
-    if (o < ((used + 15) & ~size_t(15))) return false;
-    scratch_lo = o;
+    const size_t o = (scratch_bytes + 255) & ~size_t(255);
+    if (o + n + host.size() + 256 > ProgramRing::kMaxBytes) return false;
+    scratch_bytes = o + n;
     *off = o;
     return true;
   }
@@ -426,15 +503,42 @@ struct Program {
   T* dev(size_t off) const {
     return reinterpret_cast<T*>(ctx->ring.d[slot] + off);
   }
+  template <typename T>
+  T* scratch(size_t off) const {
+    return reinterpret_cast<T*>(ctx->ring.d[slot] + scratch_base + off);
+  }
   void commit() {
-    if (used) HIP_CHECK(hipMemcpyAsync(ctx->ring.d[slot], ctx->ring.h[slot], used,
-                                       hipMemcpyHostToDevice, stream));
+    ProgramRing& R = ctx->ring;
+    const size_t used = host.size();
+    scratch_base = (used + 255) & ~size_t(255);
+    const size_t need_d = scratch_base + scratch_bytes;
+    // the slot's previous use has completed, so its buffers can be replaced
+    if (R.cap_h[slot] < used) {
+      if (R.h[slot]) HIP_CHECK(hipHostFree(R.h[slot]));
+      R.h[slot] = nullptr;
+      R.cap_h[slot] = 0;
+      const size_t cap = std::max(ProgramRing::kMinBytes, pow2_ceil(used));
+      HIP_CHECK(hipHostMalloc((void**)&R.h[slot], cap, hipHostMallocDefault));
+      R.cap_h[slot] = cap;
+    }
+    if (R.cap_d[slot] < need_d || !R.d[slot]) {
+      if (R.d[slot]) HIP_CHECK(hipFree(R.d[slot]));
+      R.d[slot] = nullptr;
+      R.cap_d[slot] = 0;
+      const size_t cap = std::max(ProgramRing::kMinBytes, pow2_ceil(need_d));
+      HIP_CHECK(hipMalloc((void**)&R.d[slot], cap));
+      R.cap_d[slot] = cap;
+    }
+    if (used) {
+      std::memcpy(R.h[slot], host.data(), used);
+      HIP_CHECK(hipMemcpyAsync(R.d[slot], R.h[slot], used, hipMemcpyHostToDevice, stream));
+    }
     committed = true;
   }
   void finish() {
     HIP_CHECK(hipEventRecord(ctx->ring.ev[slot], stream));
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->ring.armed[slot] = true;
+    finished = true;
+    release(true);
   }
 };
 
@@ -794,7 +898,7 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     launches.push_back([=, &ctx, &prog, &win, &gr, &ds]() {
       DiagFuse dfz{};
       if (fz_on)
-        dfz = DiagFuse{prog.dev<Cx<double>>(ft.bfac_off), prog.dev<Cx<double>>(ft.rfac_off),
+        dfz = DiagFuse{prog.scratch<Cx<double>>(ft.bfac_off), prog.scratch<Cx<double>>(ft.rfac_off),
                        ft.n_fac, w};
       fiber_pass(dtype, mode, state, dst, ld, ld_dst, n_batch, ts, win, all, gr, los, gd_cap,
                  gm_cap, prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff), n_layers,
@@ -848,11 +952,11 @@ bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
       const long n_a = dim_a * nf;
       hipLaunchKernelGGL(diag_row_factor_kernel, dim3((unsigned)((n_a + 255) / 256)), dim3(256),
                          0, stream, (int)dim_a, sa.d, prog.dev<DiagHeader>(hoff),
-                         prog.dev<double>(coff), prog.dev<Cx<double>>(ft.bfac_off));
+                         prog.dev<double>(coff), prog.scratch<Cx<double>>(ft.bfac_off));
       HIP_CHECK(hipGetLastError());
       hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((dim_b + 255) / 256)), dim3(256),
                          0, stream, (int)dim_b, sb.d, prog.dev<DiagHeader>(hoff),
-                         prog.dev<double>(coff), prog.dev<double>(ft.rfac_off), 1);
+                         prog.dev<double>(coff), prog.scratch<double>(ft.rfac_off), 1);
       HIP_CHECK(hipGetLastError());
     });
   }
@@ -891,11 +995,13 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   const auto [hoff, coff] = append_diag_params(prog, spec);
   // per column: gamma (contractions) or its phase factor (cos, sin) (evolution)
   const int evolve = op == DIAG_EVOLVE ? 1 : 0;
-  const size_t goff = prog.append(nullptr, (size_t)n_cols * (evolve ? 2 : 1) * sizeof(double));
+  size_t goff = 0;
+  require(prog.reserve((size_t)n_cols * (evolve ? 2 : 1) * sizeof(double), &goff),
+          FFSIM_ERR_VALUE, "diagonal operator: too many columns");
   prog.commit();
   const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
   const double* dc = prog.dev<double>(coff);
-  double* dg = prog.dev<double>(goff);
+  double* dg = prog.scratch<double>(goff);
   hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((n_cols + 255) / 256)), dim3(256), 0,
                      stream, (int)n_cols, cs.d, dh, dc, dg, evolve);
   HIP_CHECK(hipGetLastError());
@@ -991,6 +1097,49 @@ std::vector<cdouble> matmul(const cdouble* A, const cdouble* B, int n, bool adj_
   return C;
 }
 
+
+// Launches a set of row-pair (or column-pair) rotations; every rotation of the
+// set must touch disjoint rows (columns).
+void launch_pair_rotations(int dtype, void* vec, long n_rows, long n_cols, bool on_rows,
+                           const PairRotationSet& set, int n_rot, hipStream_t stream) {
+  long max_pairs = 0;
+  for (int r = 0; r < n_rot; ++r) max_pairs = std::max(max_pairs, set.r[r].n_pairs);
+  if (n_rot == 0 || max_pairs == 0 || n_rows == 0 || n_cols == 0) return;
+  if (on_rows) {
+    const dim3 grid((unsigned)std::min<long>(max_pairs, 65535), (unsigned)n_rot);
+    if (dtype == FFSIM_C128)
+      hipLaunchKernelGGL(givens_rows_kernel<double>, grid, dim3(256), 0, stream,
+                         (Cx<double>*)vec, n_cols, set);
+    else
+      hipLaunchKernelGGL(givens_rows_kernel<float>, grid, dim3(256), 0, stream,
+                         (Cx<float>*)vec, n_cols, set);
+  } else {
+    const dim3 grid((unsigned)((max_pairs + 255) / 256), (unsigned)std::min<long>(n_rows, 1024),
+                    (unsigned)n_rot);
+    require(grid.x <= 0x7fffffffu, FFSIM_ERR_VALUE, "too many pairs");
+    if (dtype == FFSIM_C128)
+      hipLaunchKernelGGL(givens_cols_kernel<double>, grid, dim3(256), 0, stream,
+                         (Cx<double>*)vec, n_rows, n_cols, set);
+    else
+      hipLaunchKernelGGL(givens_cols_kernel<float>, grid, dim3(256), 0, stream,
+                         (Cx<float>*)vec, n_rows, n_cols, set);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// Copies a host index list (ffsim passes usize arrays) into the call's
+// program as int64, after checking every entry against the dimension.
+size_t append_indices(Program& prog, const uint64_t* idx, int64_t n, int64_t bound,
+                      const char* what) {
+  std::vector<int64_t> v((size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    require(idx[k] < (uint64_t)bound, FFSIM_ERR_VALUE,
+            std::string(what) + ": index " + std::to_string(idx[k]) + " out of bounds for " +
+                std::to_string(bound) + " rows");
+    v[(size_t)k] = (int64_t)idx[k];
+  }
+  return prog.append(v.data(), v.size() * sizeof(int64_t));
+}
 }  // namespace
 }  // namespace ffsim_amd
 
@@ -1402,6 +1551,88 @@ int ffsim_apply_diag_coulomb_sequence(void* vec, void* work, int dtype, int norb
       for (int k = 0; k < 3; ++k) m[3 * t + k] = mats + (3 * t + k) * nn;
     diag_coulomb_sequence_impl(ctx, vec, work, dtype, norb, n_alpha, n_beta, n_terms, rots.data(),
                                m.data(), times, z_representation, (hipStream_t)stream);
+  });
+}
+
+int ffsim_apply_givens_rotation_in_place(void* vec, int dtype, int64_t n_rows, int64_t n_cols,
+                                         double c, double s_re, double s_im,
+                                         const uint64_t* slice1, const uint64_t* slice2,
+                                         int64_t n_pairs, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_rows >= 0 && n_cols >= 0 && n_pairs >= 0, FFSIM_ERR_VALUE, "negative size");
+    if (n_pairs == 0 || n_cols == 0) return;
+    Context& ctx = context();
+    hipStream_t st = (hipStream_t)stream;
+    Program prog(ctx, st);
+    const size_t o1 = append_indices(prog, slice1, n_pairs, n_rows, "slice1");
+    const size_t o2 = append_indices(prog, slice2, n_pairs, n_rows, "slice2");
+    prog.commit();
+    PairRotationSet set{};
+    set.r[0] = PairRotation{prog.dev<int64_t>(o1), prog.dev<int64_t>(o2), (long)n_pairs, c, s_re,
+                            s_im};
+    launch_pair_rotations(dtype, vec, n_rows, n_cols, true, set, 1, st);
+    prog.finish();
+  });
+}
+
+int ffsim_apply_phase_shift_in_place(void* vec, int dtype, int64_t n_rows, int64_t n_cols,
+                                     double phase_re, double phase_im, const uint64_t* indices,
+                                     int64_t n_indices, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(n_rows >= 0 && n_cols >= 0 && n_indices >= 0, FFSIM_ERR_VALUE, "negative size");
+    if (n_indices == 0 || n_cols == 0) return;
+    Context& ctx = context();
+    hipStream_t st = (hipStream_t)stream;
+    Program prog(ctx, st);
+    const size_t oi = append_indices(prog, indices, n_indices, n_rows, "indices");
+    prog.commit();
+    const unsigned grid = (unsigned)std::min<int64_t>(n_indices, 65535);
+    if (dtype == FFSIM_C128)
+      hipLaunchKernelGGL(phase_rows_kernel<double>, dim3(grid), dim3(256), 0, st,
+                         (Cx<double>*)vec, (long)n_cols, prog.dev<int64_t>(oi), (long)n_indices,
+                         phase_re, phase_im);
+    else
+      hipLaunchKernelGGL(phase_rows_kernel<float>, dim3(grid), dim3(256), 0, st,
+                         (Cx<float>*)vec, (long)n_cols, prog.dev<int64_t>(oi), (long)n_indices,
+                         phase_re, phase_im);
+    HIP_CHECK(hipGetLastError());
+    prog.finish();
+  });
+}
+
+int ffsim_apply_givens_layer(void* vec, int dtype, int norb, int nocc, int64_t n_other,
+                             int strings_on_rows, int n_rot, const double* c, const double* s,
+                             const int32_t* i, const int32_t* j, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    require(norb >= 0 && norb <= 63 && nocc >= 0 && n_other >= 0 && n_rot >= 0,
+            FFSIM_ERR_VALUE, "bad sizes");
+    require(n_rot <= kMaxLayerRotations, FFSIM_ERR_VALUE, "too many rotations in one layer");
+    uint64_t used = 0;
+    for (int r = 0; r < n_rot; ++r) {
+      require(i[r] >= 0 && j[r] >= 0 && i[r] < norb && j[r] < norb, FFSIM_ERR_VALUE,
+              "orbital index out of range");
+      require(i[r] == j[r] + 1 || j[r] == i[r] + 1, FFSIM_ERR_ASSERT,
+              "Target orbitals must be adjacent.");
+      const uint64_t m = (1ull << i[r]) | (1ull << j[r]);
+      require(!(used & m), FFSIM_ERR_VALUE, "rotations of a layer must act on disjoint orbitals");
+      used |= m;
+    }
+    const long dim = (long)binom(norb, nocc);
+    if (n_rot == 0 || dim == 0 || n_other == 0) return;
+    Context& ctx = context();
+    PairRotationSet set{};
+    for (int r = 0; r < n_rot; ++r) {
+      const DevPairs& dp = dev_pairs(ctx, norb, nocc, i[r], j[r]);
+      set.r[r] = PairRotation{dp.first, dp.second, dp.n, c[r], s[2 * r], s[2 * r + 1]};
+    }
+    if (strings_on_rows)
+      launch_pair_rotations(dtype, vec, dim, (long)n_other, true, set, n_rot, (hipStream_t)stream);
+    else
+      launch_pair_rotations(dtype, vec, (long)n_other, dim, false, set, n_rot,
+                            (hipStream_t)stream);
   });
 }
 

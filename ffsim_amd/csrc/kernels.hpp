@@ -1137,3 +1137,76 @@ __global__ void axpby_kernel(Cx<T>* out, const Cx<T>* x,
 }
 
 }  // namespace ffsim_amd
+
+namespace ffsim_amd {
+
+// ---------------------------------------------------------------------------
+// Single Givens rotations and phase shifts on explicit index lists, the
+// granularity of ffsim's own native calls (one rotation of row pairs per call).
+// Rotation k of a launch pairs first[p] with second[p] for p < n_pairs and
+// applies x' = c x + s y, y' = c y - conj(s) x to the pair.
+struct PairRotation {
+  const int64_t* first;
+  const int64_t* second;
+  long n_pairs;
+  double c, sr, si;
+};
+// The rotations of one launch, passed by value (a layer of disjoint rotations
+// on norb <= 64 orbitals has at most 32).
+constexpr int kMaxLayerRotations = 32;
+struct PairRotationSet {
+  PairRotation r[kMaxLayerRotations];
+};
+
+// Strings index rows of an (n_rows x n_cols) matrix: one block per row pair at a
+// time, threads along the row (coalesced).  blockIdx.y selects the rotation.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    givens_rows_kernel(Cx<T>* __restrict__ vec, long n_cols, PairRotationSet rots) {
+  const PairRotation R = rots.r[blockIdx.y];
+  const T c = (T)R.c, sr = (T)R.sr, si = (T)R.si;
+  for (long p = blockIdx.x; p < R.n_pairs; p += gridDim.x) {
+    Cx<T>* x = vec + R.first[p] * n_cols;
+    Cx<T>* y = vec + R.second[p] * n_cols;
+    for (long k = threadIdx.x; k < n_cols; k += 256) {
+      Cx<T> a = x[k], b = y[k];
+      rot_pair<T>(a, b, c, sr, si);
+      x[k] = a;
+      y[k] = b;
+    }
+  }
+}
+
+// Strings index columns: one thread per (pair, row), rows strided over gridDim.y.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    givens_cols_kernel(Cx<T>* __restrict__ vec, long n_rows, long n_cols,
+                       PairRotationSet rots) {
+  const PairRotation R = rots.r[blockIdx.z];
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= R.n_pairs) return;
+  const long cx = R.first[p], cy = R.second[p];
+  const T c = (T)R.c, sr = (T)R.sr, si = (T)R.si;
+  for (long r = blockIdx.y; r < n_rows; r += gridDim.y) {
+    Cx<T> a = vec[r * n_cols + cx], b = vec[r * n_cols + cy];
+    rot_pair<T>(a, b, c, sr, si);
+    vec[r * n_cols + cx] = a;
+    vec[r * n_cols + cy] = b;
+  }
+}
+
+// Rows idx[k] of an (n_rows x n_cols) matrix multiplied by a phase.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    phase_rows_kernel(Cx<T>* __restrict__ vec, long n_cols, const int64_t* __restrict__ idx,
+                      long n_idx, double pr, double pi) {
+  for (long q = blockIdx.x; q < n_idx; q += gridDim.x) {
+    Cx<T>* row = vec + idx[q] * n_cols;
+    for (long k = threadIdx.x; k < n_cols; k += 256) {
+      const Cx<T> a = row[k];
+      row[k] = Cx<T>{(T)(a.re * pr - a.im * pi), (T)(a.re * pi + a.im * pr)};
+    }
+  }
+}
+
+}  // namespace ffsim_amd

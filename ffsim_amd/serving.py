@@ -149,10 +149,19 @@ class DeviceBackend:
         return handle[1].query()
 
     def collect(self, req: LUCJRequest, handle):
+        """The result: a numpy vector, or (``to_host=False``) the device tensor.
+
+        A device result was allocated on the server's side stream and is complete
+        when returned.  It is marked as used by the device's default stream, so
+        the caching allocator does not hand its memory to another request while
+        work queued there still reads it; a caller that reads it on yet another
+        stream calls ``tensor.record_stream(that_stream)`` first.
+        """
         t, ev, stream, staged = handle
         ev.synchronize()
         if staged is not None:
             return self._device.finish_device_to_host(staged)
+        t.record_stream(self.torch.cuda.default_stream(self.device))
         return t
 
     def release(self, handle) -> None:

@@ -181,6 +181,41 @@ int ffsim_apply_diag_coulomb_sequence(void* vec, void* work, int dtype, int norb
                                       const double* mats, const double* times,
                                       int z_representation, void* stream);
 
+/* ---- ffsim's native calls one for one (device state, one rotation or one
+ * phase per call, as ffsim's own Python schedule issues them) ------------- */
+
+/* Replaces _lib.apply_givens_rotation_in_place (src/gates/orbital_rotation.rs:19-104,
+ * python/ffsim/_lib.pyi:39-41).  vec: DEVICE n_rows x n_cols state; slice1 /
+ * slice2: HOST arrays of n_pairs row indices (ffsim's usize tables).  For each
+ * pair (r1, r2): r1' = c r1 + s r2, r2' = c r2 - conj(s) r1.  Rows must be
+ * distinct across pairs (as ffsim's tables are).  An index >= n_rows is an
+ * error (ffsim panics). */
+int ffsim_apply_givens_rotation_in_place(void* vec, int dtype, int64_t n_rows, int64_t n_cols,
+                                         double c, double s_re, double s_im,
+                                         const uint64_t* slice1, const uint64_t* slice2,
+                                         int64_t n_pairs, void* stream);
+
+/* Replaces _lib.apply_phase_shift_in_place (src/gates/phase_shift.rs:16-30,
+ * python/ffsim/_lib.pyi:36-38): rows indices[k] of the DEVICE n_rows x n_cols
+ * state are multiplied by phase.  indices: HOST array of n_indices. */
+int ffsim_apply_phase_shift_in_place(void* vec, int dtype, int64_t n_rows, int64_t n_cols,
+                                     double phase_re, double phase_im, const uint64_t* indices,
+                                     int64_t n_indices, void* stream);
+
+/* One layer of disjoint adjacent Givens rotations on one spin sector, in one
+ * pass: the n_rot calls of _lib.apply_givens_rotation_in_place that
+ * python/ffsim/gates/orbital_rotation.py:132-135 would issue for these
+ * rotations, with the pair tables of gates/orbital_rotation.py:202-212 built
+ * and cached on the device.  Rotation r applies (c[r], s[2r] + i s[2r+1]) to
+ * the strings with orbital i[r] occupied / j[r] empty (first) and their partners
+ * (second) exactly as ffsim's call does; |i - j| must be 1 (AssertionError
+ * otherwise, gates/orbital_rotation.py:195) and the orbital pairs disjoint.
+ * strings_on_rows = 1: vec is C(norb, nocc) x n_other (the alpha sector);
+ * 0: vec is n_other x C(norb, nocc) (the beta sector in place, no transpose). */
+int ffsim_apply_givens_layer(void* vec, int dtype, int norb, int nocc, int64_t n_other,
+                             int strings_on_rows, int n_rot, const double* c, const double* s,
+                             const int32_t* i, const int32_t* j, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -915,17 +915,23 @@ def molecular_hamiltonian_dense(h1, h2, constant, norb, nelec) -> np.ndarray:
     return H
 
 
-def diag_coulomb_energies(mat, norb, nelec, z_representation=False) -> np.ndarray:
+def diag_coulomb_energies(mat, norb, nelec, z_representation=False, rows=None) -> np.ndarray:
     """E(a, b) on every determinant from occupation numbers (closed form).
 
     Number representation: sum_{ij, sigma tau} J^{sigma tau}_ij n_i n_j / 2 with
     J given as one matrix or (aa, ab, bb) (None -> 0).  Z representation:
     (1/8) sum^*_{ij sigma tau} J z_i z_j with z = 1 - 2 n, the starred sum
     excluding i == j within one spin.
+
+    With ``rows`` (a slice of alpha addresses) only that block is formed and it
+    is returned as a 2-D (alpha rows x dim_b) array; otherwise the whole state's
+    energies, flattened.
     """
     na, nb = nelec
     occ_a = occupations(colex_strings(norb, na), norb)
     occ_b = occupations(colex_strings(norb, nb), norb)
+    if rows is not None:
+        occ_a = occ_a[rows]
     if isinstance(mat, np.ndarray):
         aa = ab = bb = mat
         ba = mat
@@ -936,14 +942,16 @@ def diag_coulomb_energies(mat, norb, nelec, z_representation=False) -> np.ndarra
         ea = 0.5 * np.einsum("ai,ij,aj->a", occ_a, aa, occ_a)
         eb = 0.5 * np.einsum("bi,ij,bj->b", occ_b, bb, occ_b)
         cross = 0.5 * (occ_a @ ab @ occ_b.T + (occ_b @ ba @ occ_a.T).T)
-        return (ea[:, None] + eb[None, :] + cross).reshape(-1)
+        e = ea[:, None] + eb[None, :] + cross
+        return e if rows is not None else e.reshape(-1)
     za, zb = 1 - 2 * occ_a, 1 - 2 * occ_b
     off_aa = aa - np.diag(np.diag(aa))
     off_bb = bb - np.diag(np.diag(bb))
     ea = np.einsum("ai,ij,aj->a", za, off_aa, za)
     eb = np.einsum("bi,ij,bj->b", zb, off_bb, zb)
     cross = za @ ab @ zb.T + (zb @ ba @ za.T).T
-    return (0.125 * (ea[:, None] + eb[None, :] + cross)).reshape(-1)
+    e = 0.125 * (ea[:, None] + eb[None, :] + cross)
+    return e if rows is not None else e.reshape(-1)
 
 
 def num_op_energies(coeffs_a, coeffs_b, norb, nelec) -> np.ndarray:

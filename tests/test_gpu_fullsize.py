@@ -1,0 +1,279 @@
+"""Oracle parity at the full benchmark shapes: norb=16 (8,8) and norb=18 (9,9).
+
+The oracle cannot rotate a 12870 x 12870 (or 48620 x 48620) state in test
+time, but a product state psi = a (x) b stays a product under an orbital
+rotation: Lambda(U_a) (x) Lambda(U_b) (a (x) b) = (Lambda(U_a) a) (x)
+(Lambda(U_b) b).  Each factor is one spin sector, a 12870- or 48620-long
+vector that the oracle rotates in milliseconds.  So the GPU's full-size
+window schedules (3 passes per sector at norb=16, 4 at norb=18), the
+transposed beta passes and the fused diagonal store are compared element by
+element with the oracle:
+
+* the rotation with distinct U_a, U_b (a swapped or transposed sector, or
+  Lambda(conj U) / Lambda(U^T), shows up at once);
+* the rotation followed by a diagonal Coulomb evolution through the gate
+  sequence entry point (the evolution is fused into the last beta pass),
+  against the oracle's closed-form eigenphases exp(-i t E(a, b)) formed on
+  the host in blocks of rows;
+* diag-Coulomb evolution alone (distinct aa, bb and a non-symmetric ab, so
+  E(a, b) != E(b, a)) and num-op-sum evolution (which factorizes over the
+  sectors) on the same product states.
+
+Bounds: complex128 |delta|_inf <= 1e-12 and 1 - |<psi_ref|psi>| <= 1e-12;
+complex64 |delta|_inf <= 2e-5 max|psi_ref| and 1 - |<psi_ref|psi>| <= 1e-9
+(FP32 storage, FP64 coefficients).  Expected values and overlaps are formed
+in complex128.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C128_TOL = 1e-12
+C128_INFID = 1e-12
+C64_REL = 2e-5
+C64_INFID = 1e-9
+
+
+@pytest.fixture(autouse=True)
+def _release_cached_blocks():
+    """Return each test's multi-GB blocks to the device between tests."""
+    yield
+    import torch
+
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+
+def _case(*args):
+    print("CASE", *args, flush=True)
+    sys.stdout.flush()
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    return torch
+
+
+def _product_state(a, b, dtype):
+    """psi[i, j] = a[i] b[j] as a flat device tensor of the given dtype."""
+    torch = _torch()
+    cdt = torch.complex128 if dtype == "c128" else torch.complex64
+    ta = torch.from_numpy(a).to("cuda").to(cdt)
+    tb = torch.from_numpy(b).to("cuda").to(cdt)
+    return (ta[:, None] * tb[None, :]).reshape(-1)
+
+
+def _compare(t, ra, rb, dim_a, dim_b, phase_rows=None, chunk=1024):
+    """(max |psi - ref|, max |ref|, 1 - |<ref|psi>|) with ref[i, j] = ra[i] rb[j]
+    (times phase_rows(i0, i1)[i - i0, j] when given), blocks of rows at a time."""
+    torch = _torch()
+    psi = t.reshape(dim_a, dim_b)
+    ta = torch.from_numpy(ra).to("cuda")
+    tb = torch.from_numpy(rb).to("cuda")
+    max_err = 0.0
+    max_ref = 0.0
+    overlap = 0.0 + 0.0j
+    norm_ref = 0.0
+    norm_psi = 0.0
+    for i0 in range(0, dim_a, chunk):
+        i1 = min(dim_a, i0 + chunk)
+        ref = ta[i0:i1, None] * tb[None, :]
+        if phase_rows is not None:
+            ref = ref * torch.from_numpy(phase_rows(i0, i1)).to("cuda")
+        got = psi[i0:i1].to(torch.complex128)
+        max_err = max(max_err, float((got - ref).abs().max()))
+        max_ref = max(max_ref, float(ref.abs().max()))
+        overlap += complex((ref.conj() * got).sum())
+        norm_ref += float((ref.abs() ** 2).sum())
+        norm_psi += float((got.abs() ** 2).sum())
+    infid = 1.0 - abs(overlap) / math.sqrt(norm_ref * norm_psi)
+    return max_err, max_ref, infid
+
+
+def _check(dtype, err, ref_max, infid):
+    print(f"  max|delta|={err:.3e} max|ref|={ref_max:.3e} infidelity={infid:.3e}", flush=True)
+    if dtype == "c128":
+        assert err <= C128_TOL, err
+        assert infid <= C128_INFID, infid
+    else:
+        assert err <= C64_REL * ref_max, (err, ref_max)
+        assert infid <= C64_INFID, infid
+
+
+def _factors(norb, nelec, rng):
+    na, nb = nelec
+    a = O.random_state_vector(math.comb(norb, na), rng)
+    b = O.random_state_vector(math.comb(norb, nb), rng)
+    return a, b
+
+
+def _rotate_sector(v, U, norb, nocc):
+    return O.apply_orbital_rotation(v, U, norb, nocc)
+
+
+def _phase_block(mats, time, norb, nelec):
+    """exp(-i t E(a, b)) for alpha rows [i0, i1), from the oracle's closed form."""
+
+    def block(i0, i1):
+        e = O.diag_coulomb_energies(mats, norb, nelec, rows=slice(i0, i1))
+        return np.exp(-1j * time * e)
+
+    return block
+
+
+@pytest.mark.parametrize(
+    "norb,nelec,dtype",
+    [(16, (8, 8), "c128"), (16, (8, 8), "c64"), (18, (9, 9), "c128"), (18, (9, 9), "c64")],
+)
+def test_product_state_rotation(norb, nelec, dtype):
+    import ffsim_amd
+
+    rng = np.random.default_rng(norb * 100 + (dtype == "c64"))
+    a, b = _factors(norb, nelec, rng)
+    ua, ub = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+    ra = _rotate_sector(a, ua, norb, nelec[0])
+    rb = _rotate_sector(b, ub, norb, nelec[1])
+    da, db = len(a), len(b)
+    _case("product_state_rotation", norb, nelec, dtype, da, db)
+    t = _product_state(a, b, dtype)
+    ffsim_amd.apply_orbital_rotation(t, (ua, ub), norb, nelec, copy=False)
+    _check(dtype, *_compare(t, ra, rb, da, db))
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_product_state_rotation_then_fused_diag_coulomb_norb16(dtype):
+    """Rotation then diag-Coulomb evolution (J symmetric U(-1,1), t=1.0) through
+    the sequence entry point: in c128 the evolution runs inside the last
+    (transposing) beta pass."""
+    from ffsim_amd import _device, _lib
+
+    norb, nelec = 16, (8, 8)
+    rng = np.random.default_rng(1600 + (dtype == "c64"))
+    a, b = _factors(norb, nelec, rng)
+    u = O.random_unitary(norb, rng)
+    J = O.random_symmetric_uniform(norb, rng)
+    time = 1.0
+    ra = _rotate_sector(a, u, norb, nelec[0])
+    rb = _rotate_sector(b, u, norb, nelec[1])
+    da, db = len(a), len(b)
+    _case("rotation_then_fused_diag", norb, nelec, dtype)
+    t = _product_state(a, b, dtype)
+    work = _device.work_buffer(t, norb, *nelec)
+    rots = np.ascontiguousarray(np.stack([u, np.eye(norb, dtype=complex)]))
+    jh = J.copy()
+    jh[np.diag_indices(norb)] *= 0.5
+    mats = np.ascontiguousarray(np.stack([jh, J, jh])[None])
+    times = np.array([time])
+    _lib.call("ffsim_apply_diag_coulomb_sequence", _device.ptr(t), _device.ptr(work),
+              _device.dtype_tag(t), norb, nelec[0], nelec[1], 1, _device.host_ptr(rots),
+              _device.host_ptr(mats), _device.host_ptr(times), 0, _device.stream_ptr())
+    _check(dtype, *_compare(t, ra, rb, da, db, _phase_block(J, time, norb, nelec)))
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_diag_coulomb_evolution_norb16(dtype):
+    import ffsim_amd
+
+    norb, nelec = 16, (8, 8)
+    rng = np.random.default_rng(1610 + (dtype == "c64"))
+    a, b = _factors(norb, nelec, rng)
+    aa = O.random_symmetric_uniform(norb, rng)
+    bb = O.random_symmetric_uniform(norb, rng)
+    ab = rng.uniform(-1, 1, size=(norb, norb))  # not symmetric: E(a, b) != E(b, a)
+    time = 1.0
+    _case("diag_coulomb_evolution", norb, nelec, dtype)
+    t = _product_state(a, b, dtype)
+    ffsim_amd.apply_diag_coulomb_evolution(t, (aa, ab, bb), time, norb, nelec, copy=False)
+    _check(dtype, *_compare(t, a, b, len(a), len(b),
+                            _phase_block((aa, ab, bb), time, norb, nelec)))
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("pair", [False, True])
+def test_num_op_sum_evolution_norb16(dtype, pair):
+    """Config 2's num-op-sum leg (one random real 16-vector, U(-1,1), t=1.0), and
+    distinct alpha / beta coefficients."""
+    import ffsim_amd
+
+    norb, nelec = 16, (8, 8)
+    rng = np.random.default_rng(1620 + 2 * pair + (dtype == "c64"))
+    a, b = _factors(norb, nelec, rng)
+    ca = rng.uniform(-1, 1, size=norb)
+    cb = rng.uniform(-1, 1, size=norb) if pair else ca
+    time = 1.0
+    ea = a * np.exp(-1j * time * O.num_op_energies(ca, None, norb, (nelec[0], 0)))
+    eb = b * np.exp(-1j * time * O.num_op_energies(cb, None, norb, (nelec[1], 0)))
+    _case("num_op_sum_evolution", norb, nelec, dtype, pair)
+    t = _product_state(a, b, dtype)
+    coeffs = (ca, cb) if pair else ca
+    ffsim_amd.apply_num_op_sum_evolution(t, coeffs, time, norb, nelec, copy=False)
+    _check(dtype, *_compare(t, ea, eb, len(a), len(b)))
+
+
+def test_norb18_c64_trotter_step_error_vs_c128_and_round_trip():
+    """Config 5's complex64 leg: one DF Trotter step at norb=18 (9,9) (one-body
+    term plus 10 factors, J ~ N(0, 0.1), t=0.1) from the same product state in
+    c64 and c128.  The c64 result's norm and its error against the c128 result
+    are bounded, and the c64 step followed by its inverse (the gate sequence
+    reversed, rotations conjugated, times negated) returns the state."""
+    import torch
+
+    from ffsim_amd import random as frandom
+    from ffsim_amd import trotter
+
+    norb, nelec = 18, (9, 9)
+    rng = np.random.default_rng(1800)
+    ham = frandom.random_df_hamiltonian_normal(norb, n_factors=10, seed=rng)
+    a, b = _factors(norb, nelec, rng)
+    da, db = len(a), len(b)
+    rots, mats, times = trotter.trotter_gate_sequence(ham, 0.1, norb, 1, 0)
+    mats = [np.asarray(m, dtype=float) for m in mats]
+    _case("norb18_trotter_c64_vs_c128", norb, nelec, len(mats))
+    t64 = _product_state(a, b, "c64")
+    t128 = _product_state(a, b, "c128")
+    trotter._run_sequence(t64, rots, mats, times, False, norb, nelec)
+    trotter._run_sequence(t128, rots, mats, times, False, norb, nelec)
+    max_err = 0.0
+    max_ref = 0.0
+    overlap = 0.0 + 0.0j
+    n64 = 0.0
+    n128 = 0.0
+    chunk = 1024
+    v64 = t64.reshape(da, db)
+    v128 = t128.reshape(da, db)
+    for i0 in range(0, da, chunk):
+        g = v64[i0:i0 + chunk].to(torch.complex128)
+        r = v128[i0:i0 + chunk]
+        max_err = max(max_err, float((g - r).abs().max()))
+        max_ref = max(max_ref, float(r.abs().max()))
+        overlap += complex((r.conj() * g).sum())
+        n64 += float((g.abs() ** 2).sum())
+        n128 += float((r.abs() ** 2).sum())
+    infid = 1.0 - abs(overlap) / math.sqrt(n64 * n128)
+    print(f"  c64 vs c128: max|delta|={max_err:.3e} (max|psi| {max_ref:.3e}) "
+          f"infidelity={infid:.3e} |psi64|-1={math.sqrt(n64) - 1:.3e}", flush=True)
+    assert abs(math.sqrt(n128) - 1) <= 1e-12
+    assert abs(math.sqrt(n64) - 1) <= 1e-5
+    assert max_err <= C64_REL * max_ref
+    assert infid <= C64_INFID
+    del t128, v128
+    torch.cuda.empty_cache()
+    _case("norb18_trotter_c64_round_trip")
+    inv_rots = [np.asarray(r).T.conj() for r in reversed(rots)]
+    trotter._run_sequence(t64, inv_rots, mats[::-1], [-x for x in times[::-1]], False, norb,
+                          nelec)
+    _check("c64", *_compare(t64, a, b, da, db))
+    del t64, v64
+    torch.cuda.empty_cache()

@@ -198,3 +198,28 @@ def test_diag_and_trace_protocols(rng):
         ffsim_amd.diag(object(), norb, nelec)
     with pytest.raises(TypeError):
         ffsim_amd.trace(object(), norb, nelec)
+
+
+def test_givens_layer_argument_errors_without_gpu():
+    """The layer entry checks its rotations before touching the device: a
+    non-adjacent pair is ffsim's AssertionError (gates/orbital_rotation.py:195),
+    overlapping or out-of-range pairs are ValueErrors."""
+    lib = _lib.lib()
+
+    def call(i, j):
+        n = len(i)
+        c = np.full(n, 0.6)
+        s = np.full(n, 0.8 + 0j)
+        ii = np.asarray(i, dtype=np.int32)
+        jj = np.asarray(j, dtype=np.int32)
+        return _lib.check(lib.ffsim_apply_givens_layer(
+            None, _lib.C128, 6, 3, 10, 1, n, c.ctypes.data_as(ctypes.c_void_p),
+            s.ctypes.data_as(ctypes.c_void_p), ii.ctypes.data_as(ctypes.c_void_p),
+            jj.ctypes.data_as(ctypes.c_void_p), None))
+
+    with pytest.raises(AssertionError, match="adjacent"):
+        call([0], [2])
+    with pytest.raises(ValueError, match="disjoint"):
+        call([0, 1], [1, 2])
+    with pytest.raises(ValueError, match="out of range"):
+        call([5], [6])

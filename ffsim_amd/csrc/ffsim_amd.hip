@@ -1622,17 +1622,48 @@ int ffsim_apply_givens_layer(void* vec, int dtype, int norb, int nocc, int64_t n
     }
     const long dim = (long)binom(norb, nocc);
     if (n_rot == 0 || dim == 0 || n_other == 0) return;
+    require(dim < (1l << 31) && n_other < (1l << 31), FFSIM_ERR_VALUE, "dimension too large");
     Context& ctx = context();
-    PairRotationSet set{};
+    hipStream_t st = (hipStream_t)stream;
+    // Two rotations of a layer act on disjoint orbitals but share strings (0101
+    // pairs with 0110 under (0,1) and with 1001 under (2,3)), so they cannot run
+    // as independent launches over the same rows.  The window passes apply them
+    // in one fused pass; where those cannot run in place (strings on columns
+    // with a string vector too long for one tile) each rotation is one launch.
+    const bool fused = strings_on_rows || beta_in_columns(norb, dim, dtype, ctx);
+    if (fused) {
+      SectorRotation rot;
+      for (int r = 0; r < n_rot; ++r) {
+        KStep k{};
+        const cdouble sv(s[2 * r], s[2 * r + 1]);
+        // (c, s) on (first = i occupied / j empty, second = partner) as a
+        // kernel step on (p occupied / p+1 empty, partner)
+        const cdouble sigma = (j[r] == i[r] + 1) ? sv : -std::conj(sv);
+        k.p = std::min(i[r], j[r]);
+        k.c = c[r];
+        k.sr = sigma.real();
+        k.si = sigma.imag();
+        rot.steps.push_back(k);
+      }
+      Program prog(ctx, st);
+      std::vector<std::function<void()>> launches;
+      if (strings_on_rows)
+        run_sector(ctx, prog, dtype, MODE_ROWS, vec, (long)n_other, (int)n_other, norb, nocc, rot,
+                   nullptr, 0, st, launches);
+      else
+        run_sector(ctx, prog, dtype, MODE_COLS, vec, dim, (int)n_other, norb, nocc, rot, nullptr,
+                   0, st, launches);
+      prog.commit();
+      for (auto& f : launches) f();
+      prog.finish();
+      return;
+    }
     for (int r = 0; r < n_rot; ++r) {
       const DevPairs& dp = dev_pairs(ctx, norb, nocc, i[r], j[r]);
-      set.r[r] = PairRotation{dp.first, dp.second, dp.n, c[r], s[2 * r], s[2 * r + 1]};
+      PairRotationSet set{};
+      set.r[0] = PairRotation{dp.first, dp.second, dp.n, c[r], s[2 * r], s[2 * r + 1]};
+      launch_pair_rotations(dtype, vec, (long)n_other, dim, false, set, 1, st);
     }
-    if (strings_on_rows)
-      launch_pair_rotations(dtype, vec, dim, (long)n_other, true, set, n_rot, (hipStream_t)stream);
-    else
-      launch_pair_rotations(dtype, vec, (long)n_other, dim, false, set, n_rot,
-                            (hipStream_t)stream);
   });
 }
 

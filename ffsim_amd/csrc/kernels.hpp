@@ -1151,8 +1151,10 @@ struct PairRotation {
   long n_pairs;
   double c, sr, si;
 };
-// The rotations of one launch, passed by value (a layer of disjoint rotations
-// on norb <= 64 orbitals has at most 32).
+// The rotations of one launch, passed by value.  Rotations of one launch run
+// concurrently, so they must touch disjoint rows (columns): two rotations on
+// disjoint orbital pairs generally do not (they share strings), and run as
+// separate launches.
 constexpr int kMaxLayerRotations = 32;
 struct PairRotationSet {
   PairRotation r[kMaxLayerRotations];

@@ -203,10 +203,12 @@ int ffsim_apply_phase_shift_in_place(void* vec, int dtype, int64_t n_rows, int64
                                      int64_t n_indices, void* stream);
 
 /* One layer of disjoint adjacent Givens rotations on one spin sector, in one
- * pass: the n_rot calls of _lib.apply_givens_rotation_in_place that
+ * call: the n_rot calls of _lib.apply_givens_rotation_in_place that
  * python/ffsim/gates/orbital_rotation.py:132-135 would issue for these
- * rotations, with the pair tables of gates/orbital_rotation.py:202-212 built
- * and cached on the device.  Rotation r applies (c[r], s[2r] + i s[2r+1]) to
+ * rotations (pair tables of gates/orbital_rotation.py:202-212).  The layer runs
+ * as fused window passes over the state when the strings index rows, or index
+ * columns and a whole string vector fits one on-chip tile; otherwise as one
+ * launch per rotation over pair tables cached on the device.  Rotation r applies (c[r], s[2r] + i s[2r+1]) to
  * the strings with orbital i[r] occupied / j[r] empty (first) and their partners
  * (second) exactly as ffsim's call does; |i - j| must be 1 (AssertionError
  * otherwise, gates/orbital_rotation.py:195) and the orbital pairs disjoint.

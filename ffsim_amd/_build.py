@@ -11,8 +11,8 @@ PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libffsim_amd.so"
 
-SOURCES = ["ffsim_amd.hip", "host_core.cpp"]
-HEADERS = ["kernels.hpp", "host_core.hpp"]
+SOURCES = ["ffsim_amd.hip", "noalloc.hip", "host_core.cpp"]
+HEADERS = ["kernels.hpp", "host_core.hpp", "status.hpp", "diag_params.hpp"]
 
 
 def hipcc() -> str:

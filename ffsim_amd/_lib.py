@@ -47,6 +47,7 @@ SIGNATURES: dict[str, list] = {
     "ffsim_apply_givens_rotation_in_place": [_P, _I, _I64, _I64, _D, _D, _D, _P, _P, _I64, _P],
     "ffsim_apply_phase_shift_in_place": [_P, _I, _I64, _I64, _D, _D, _P, _I64, _P],
     "ffsim_apply_givens_layer": [_P, _I, _I, _I, _I64, _I, _I, _P, _P, _P, _P, _P],
+    "ffsim_apply_diag_coulomb_evolution_noalloc": [_P, _I, _I, _I, _I, _P, _P, _P, _D, _I, _P],
 }
 
 

@@ -26,50 +26,30 @@
 #include <vector>
 
 #include "../../include/ffsim_amd.h"
+#include "diag_params.hpp"
 #include "host_core.hpp"
 #include "kernels.hpp"
+#include "status.hpp"
 
 namespace ffsim_amd {
 
 // ------------------------------------------------------------------ errors
 namespace {
 thread_local std::string g_last_error;
-
-struct Failure {
-  int code;
-  std::string msg;
-};
-
-#define HIP_CHECK(expr)                                                               \
-  do {                                                                                \
-    hipError_t _e = (expr);                                                           \
-    if (_e != hipSuccess)                                                             \
-      throw Failure{FFSIM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
-  } while (0)
-
-void require(bool ok, int code, const std::string& msg) {
-  if (!ok) throw Failure{code, msg};
-}
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    f();
-    return FFSIM_OK;
-  } catch (const Failure& e) {
-    g_last_error = e.msg;
-    return e.code;
-  } catch (const std::invalid_argument& e) {
-    g_last_error = e.what();
-    return FFSIM_ERR_VALUE;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return FFSIM_ERR_VALUE;
-  }
-}
 }  // namespace
 
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// Resolves the allocation-free entry point's device scratch on the current
+// device (noalloc.hip).
+void noalloc_setup();
+
 // ------------------------------------------------------------- device state
+__global__ void fill_zero_kernel(double* __restrict__ x, long n) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+    x[k] = 0.0;
+}
+
 __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ col_str,
                                      const DiagHeader* __restrict__ hdr,
                                      const double* __restrict__ coef, double* __restrict__ gamma,
@@ -791,31 +771,12 @@ SectorRotation prepare_rotation(const cdouble* mat, int norb) {
 // transposed state there (MODE_ROWS only).
 // Diagonal operator helper.  rows/cols strings are for the state's row and
 // column sectors; matrices are given for (row sector, col sector, cross).
-struct DiagSpec {
-  int norb = 0, z_rep = 0, flags = 0;
-  double scale = 1.0;
-  std::vector<double> A_row, A_col, X, lin_row, lin_col;
-};
-
 // Appends a diagonal operator's header and coefficients to a call's program;
 // returns their offsets.
 std::pair<size_t, size_t> append_diag_params(Program& prog, const DiagSpec& spec) {
-  DiagHeader h{};
-  h.norb = spec.norb;
-  h.z_rep = spec.z_rep;
-  h.flags = spec.flags;
-  h.scale = spec.scale;
+  const DiagHeader h = diag_header(spec);
   const size_t hoff = prog.append(&h, sizeof(h));
-  const size_t nn = (size_t)spec.norb * spec.norb;
-  std::vector<double> coef(3 * nn + 2 * spec.norb, 0.0);
-  auto put = [&](const std::vector<double>& v, size_t off, size_t n) {
-    if (!v.empty()) std::copy(v.begin(), v.begin() + n, coef.begin() + off);
-  };
-  put(spec.A_row, 0, nn);
-  put(spec.A_col, nn, nn);
-  put(spec.X, 2 * nn, nn);
-  put(spec.lin_row, 3 * nn, spec.norb);
-  put(spec.lin_col, 3 * nn + spec.norb, spec.norb);
+  const std::vector<double> coef = diag_coefficients(spec);
   const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
   return {hoff, coff};
 }
@@ -1002,61 +963,24 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
   const double* dc = prog.dev<double>(coff);
   double* dg = prog.scratch<double>(goff);
-  hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((n_cols + 255) / 256)), dim3(256), 0,
-                     stream, (int)n_cols, cs.d, dh, dc, dg, evolve);
+  require(diag_grid(n_rows, n_cols).y <= 65535, FFSIM_ERR_VALUE, "too many columns");
+  launch_diag_col_term(n_cols, cs.d, dh, dc, dg, evolve, stream);
   HIP_CHECK(hipGetLastError());
-  // blocks of R rows x up to kColsPerBlock columns (many blocks for a full GPU)
-  constexpr int R = 8;
-  constexpr int kColsPerBlock = 2048;
-  const int cols_blk = (int)std::min<long>(n_cols, kColsPerBlock);
-  const dim3 grid((unsigned)((n_rows + R - 1) / R), (unsigned)((n_cols + cols_blk - 1) / cols_blk));
-  require(grid.y <= 65535, FFSIM_ERR_VALUE, "too many columns");
   const double moved = (double)elem_bytes(dtype) * (double)n_rows * (double)n_cols *
                        (op == DIAG_CONTRACT_ACC ? 3.0 : 2.0);
   const long pk = prof_start(PROF_DIAG, moved, stream);
-#define FFSIM_DIAG_LAUNCH(TT, MM)                                                           \
-  hipLaunchKernelGGL((diag_kernel<TT, R, MM>), grid, dim3(256), 0, stream,                  \
-                     reinterpret_cast<const Cx<TT>*>(src), reinterpret_cast<Cx<TT>*>(dst), \
-                     n_cols, (int)n_rows, (int)n_cols, cols_blk, rs.d, cs.d, dg, dh, dc)
-  if (dtype == FFSIM_C128) {
-    if (op == DIAG_EVOLVE) FFSIM_DIAG_LAUNCH(double, DIAG_EVOLVE);
-    else if (op == DIAG_CONTRACT) FFSIM_DIAG_LAUNCH(double, DIAG_CONTRACT);
-    else FFSIM_DIAG_LAUNCH(double, DIAG_CONTRACT_ACC);
-  } else {
-    if (op == DIAG_EVOLVE) FFSIM_DIAG_LAUNCH(float, DIAG_EVOLVE);
-    else if (op == DIAG_CONTRACT) FFSIM_DIAG_LAUNCH(float, DIAG_CONTRACT);
-    else FFSIM_DIAG_LAUNCH(float, DIAG_CONTRACT_ACC);
-  }
-#undef FFSIM_DIAG_LAUNCH
+  if (dtype == FFSIM_C128)
+    launch_diag_pass_t<double>(op, src, dst, n_rows, n_cols, rs.d, cs.d, dg, dh, dc, stream);
+  else
+    launch_diag_pass_t<float>(op, src, dst, n_rows, n_cols, rs.d, cs.d, dg, dh, dc, stream);
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
   prog.finish();
 }
 
-std::vector<double> mat_or_empty(const double* m, int n) {
-  if (!m) return {};
-  return std::vector<double>(m, m + (size_t)n * n);
-}
 std::vector<double> vec_or_empty(const double* m, int n) {
   if (!m) return {};
   return std::vector<double>(m, m + n);
-}
-
-DiagSpec diag_coulomb_spec(int norb, const double* aa, const double* ab, const double* bb,
-                           int z_rep, double scale) {
-  DiagSpec s;
-  s.norb = norb;
-  s.z_rep = z_rep ? 1 : 0;
-  s.scale = scale;
-  s.A_row = mat_or_empty(aa, norb);
-  s.A_col = mat_or_empty(bb, norb);
-  s.X = mat_or_empty(ab, norb);
-  if (z_rep) {
-    for (auto* v : {&s.A_row, &s.A_col, &s.X})
-      for (auto& x : *v) x *= 0.25;
-  }
-  s.flags = ((aa || bb) ? DF_QUAD : 0) | (ab ? DF_CROSS : 0);
-  return s;
 }
 
 void check_dtype(int dtype) {
@@ -1154,6 +1078,7 @@ int ffsim_amd_init(int device) {
   return guarded([&] {
     HIP_CHECK(hipSetDevice(device));
     context();
+    noalloc_setup();
   });
 }
 

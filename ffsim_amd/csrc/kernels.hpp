@@ -1111,10 +1111,8 @@ __global__ void __launch_bounds__(256)
   out[o] = Cx<T>{(T)r, (T)i};
 }
 
-__global__ void fill_zero_kernel(double* __restrict__ x, long n) {
-  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
-    x[k] = 0.0;
-}
+// x[0..n) = 0 (defined in ffsim_amd.hip).
+__global__ void fill_zero_kernel(double* __restrict__ x, long n);
 
 // out = alpha x + beta y over n complex entries.
 template <typename T>

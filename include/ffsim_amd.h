@@ -218,6 +218,25 @@ int ffsim_apply_givens_layer(void* vec, int dtype, int norb, int nocc, int64_t n
                              int strings_on_rows, int n_rot, const double* c, const double* s,
                              const int32_t* i, const int32_t* j, void* stream);
 
+/* ---- allocation-free variant -------------------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same result as ffsim_apply_diag_coulomb_evolution
+ * (same arguments, same kernels), with the operator's parameters, the string
+ * tables and the per-column factors in a static 8 MiB device array instead of
+ * library-allocated memory; the call itself allocates nothing.  The tables of
+ * a state must fit the array: 128 KiB + 8 (dim_a + dim_b) + 16 dim_b bytes
+ * (rounded up to 256 bytes each) <= 8 MiB, e.g. norb=20 (10,10); larger states
+ * fail with FFSIM_ERR_VALUE.  Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU race (calls on one stream are
+ * ordered).  ffsim_amd_init(device) resolves the array's address on that
+ * device; otherwise the first call on a device does. */
+int ffsim_apply_diag_coulomb_evolution_noalloc(void* vec, int dtype, int norb, int n_alpha,
+                                               int n_beta, const double* mat_aa,
+                                               const double* mat_ab, const double* mat_bb,
+                                               double time, int z_representation, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

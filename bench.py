@@ -42,23 +42,63 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int,
-                 repeats: int = 3, single_thread_rotations: int = 12) -> dict:
+def host_cpu_info() -> dict:
+    """The timing host's CPU: logical CPUs, the affinity mask, the cgroup CPU
+    quota (cgroup v2 cpu.max) and the model name."""
+    info: dict = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        info["cgroup_cpu_quota"] = None if quota == "max" else int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    info["OMP_NUM_THREADS"] = os.environ.get("OMP_NUM_THREADS")
+    return info
+
+
+def usable_cpus(info: dict) -> int:
+    """CPUs this process may run on at once: the affinity mask, capped by the
+    cgroup quota when one is set."""
+    n = info.get("affinity_cpus") or info.get("nproc") or 1
+    q = info.get("cgroup_cpu_quota")
+    if q:
+        n = min(n, max(1, int(q)))
+    return int(n)
+
+
+def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, thread_counts, repeats: int = 5,
+                 single_thread_rotations: int = 12) -> dict:
     """Time a bounded sample of the same workload with the CPU oracle.
 
-    Sample: one alpha-sector orbital rotation (the 120 Givens rotations and 16
-    phase shifts of ffsim's schedule), one transpose copy and one diag-Coulomb
-    evolution on the norb=16 state.  An application costs 10 sector rotations,
-    10 transpose copies and 4 diag-Coulomb passes on that schedule, so
-    apps/s = 1 / (10 T_rot + 10 T_transpose + 4 T_dc).  With ``threads``
-    threads: one warm-up sample, then the median of ``repeats`` samples.  On one
-    thread: the first ``single_thread_rotations`` Givens rotations (scaled to
-    the sector rotation by rotation count), one transpose and one diag-Coulomb
-    pass.
+    The oracle's C restatement of ffsim's Rust kernels is built here with
+    -march=native (as ffsim's .cargo/config.toml:2 builds ffsim) and run with
+    ffsim's schedule: one call per Givens rotation with per-call threads over
+    row pairs, one per orbital phase, a full transpose copy per beta sector
+    (python/ffsim/gates/orbital_rotation.py:117-154), and a threaded
+    diag-Coulomb pass.  Sample: one alpha-sector orbital rotation (the 120
+    Givens rotations and 16 phase shifts), one transpose copy and one
+    diag-Coulomb evolution of the norb=16 state.  ffsim's LUCJ n_reps=4
+    application (ucj_spin_balanced.py:657-696) is 5 orbital rotations, i.e. 10
+    sector rotations and 10 transpose copies, plus 4 diag-Coulomb passes, so
+    apps/s = 1 / (10 T_rot + 10 T_transpose + 4 T_dc).  For each thread count:
+    one warm-up sample, then the median of ``repeats``.  On one thread: the
+    first ``single_thread_rotations`` Givens rotations scaled to the sector
+    rotation by rotation count, one transpose, one diag-Coulomb pass.
     """
+    os.environ["FFSIM_ORACLE_NATIVE"] = "1"
     from oracle import oracle as O
 
-    O.build()
+    O.build(force=True)
     O.init_cache(norb, nelec)
     da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
     v = np.ascontiguousarray(vec_host.reshape(da, db))
@@ -88,32 +128,42 @@ def cpu_baseline(norb: int, nelec, op, vec_host: np.ndarray, threads: int,
         t_dc = time.perf_counter() - t0
         return t_rot, t_tr, t_dc
 
-    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
-    sample()  # warm-up
-    runs = [sample() for _ in range(repeats)]
-    t_rot, t_tr, t_dc = (float(np.median([r[k] for r in runs])) for k in range(3))
-    t_app = 10 * t_rot + 10 * t_tr + 4 * t_dc
-    log(f"cpu sample ({threads} threads, median of {repeats}): sector rotation {t_rot:.2f}s "
-        f"transpose {t_tr:.2f}s diag-coulomb {t_dc:.2f}s")
+    runs_by_threads = {}
+    for threads in thread_counts:
+        os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
+        sample()  # warm-up
+        runs = [sample() for _ in range(repeats)]
+        t_rot, t_tr, t_dc = (float(np.median([r[k] for r in runs])) for k in range(3))
+        runs_by_threads[threads] = (t_rot, t_tr, t_dc, 10 * t_rot + 10 * t_tr + 4 * t_dc)
+        log(f"cpu sample ({threads} threads, median of {repeats}): sector rotation {t_rot:.2f}s "
+            f"transpose {t_tr:.2f}s diag-coulomb {t_dc:.2f}s")
+    best = min(runs_by_threads, key=lambda k: runs_by_threads[k][3])
+    t_rot, t_tr, t_dc, t_app = runs_by_threads[best]
     os.environ["FFSIM_ORACLE_THREADS"] = "1"
     r1, tr1, dc1 = sample(single_thread_rotations)
     t_app1 = 10 * r1 + 10 * tr1 + 4 * dc1
     log(f"cpu sample (1 thread): sector rotation {r1:.2f}s (from {single_thread_rotations} "
         f"rotations) transpose {tr1:.2f}s diag-coulomb {dc1:.2f}s")
-    os.environ["FFSIM_ORACLE_THREADS"] = str(threads)
+    os.environ["FFSIM_ORACLE_THREADS"] = str(best)
     return {
         "value": 1.0 / t_app,
         "unit": "LUCJ applications/s",
-        "cores": threads,
+        "cores": best,
         "kind": "port",
         "sample": (
-            f"C restatement of ffsim's Rust kernels with ffsim's schedule (one FFI call per Givens "
-            f"rotation, per-call threads, transpose copies), {threads} threads, one warm-up then "
+            f"C restatement of ffsim's Rust kernels (built -march=native on this host) with "
+            f"ffsim's schedule (one call per Givens rotation with per-call threads over row "
+            f"pairs, one per orbital phase, transpose copies), {best} threads, one warm-up then "
             f"the median of {repeats}: one alpha-sector orbital rotation ({t_rot:.2f} s), one "
             f"transpose copy ({t_tr:.2f} s) and one diag-Coulomb evolution ({t_dc:.2f} s) of the "
-            f"norb=16 (8,8) state, scaled to one n_reps=4 application = 10 rotations + 10 "
-            f"transposes + 4 diag-Coulomb passes"
+            f"norb=16 (8,8) state; one n_reps=4 application = 10 sector rotations + 10 transpose "
+            f"copies + 4 diag-Coulomb passes = {t_app:.1f} s (scaled from the sample by those "
+            f"counts; not timed end to end)"
         ),
+        "by_threads": {str(k): {"sector_rotation_s": round(v_[0], 3), "transpose_s": round(v_[1], 3),
+                                "diag_coulomb_s": round(v_[2], 3),
+                                "apps_per_s": round(1.0 / v_[3], 5)}
+                       for k, v_ in runs_by_threads.items()},
         "single_thread": {
             "value": 1.0 / t_app1,
             "sample": (f"1 thread: {single_thread_rotations} of the {len(rots)} Givens rotations "
@@ -170,6 +220,127 @@ def measured_traffic(kernel: str, workload: str, dtype: str):
         if (entry.get("kernel"), entry.get("workload"), entry.get("dtype")) == (kernel, workload, dtype):
             return entry
     return None
+
+
+def _profiled(fn, reps: int) -> dict:
+    """Run fn() reps times between profiler begin/end (after one warm-up call):
+    wall ms per call and the fiber / diagonal kernels' time and algorithmic bytes."""
+    import torch
+
+    from ffsim_amd import _lib
+
+    fn()
+    torch.cuda.synchronize()
+    buf = (ctypes.c_double * 6)()
+    _lib.call("ffsim_amd_profile_begin")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    _lib.call("ffsim_amd_profile_end", buf)
+    fms, fn_, fb, dms, dn, db = list(buf)
+    out = {"ms_per_call": round(wall * 1e3, 3)}
+    if fn_:
+        out["fiber_launches_per_call"] = fn_ / reps
+        out["fiber_avg_ms"] = round(fms / fn_, 4)
+        out["fiber_GBs"] = round(fb / (fms * 1e6), 1)
+    if dn:
+        out["diag_launches_per_call"] = dn / reps
+        out["diag_avg_ms"] = round(dms / dn, 4)
+        out["diag_GBs"] = round(db / (dms * 1e6), 1)
+    return out
+
+
+def lucj_rate(op, norb: int, nelec, cdtype, steps: int, warmup: int, seed: int = 0) -> dict:
+    """LUCJ applications/s on a resident state of the given dtype (same seeded
+    state as the headline run), with the fiber kernel's average launch."""
+    import torch
+
+    from ffsim_amd import _lib
+
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    state = torch.randn(da * db, dtype=torch.complex128, device="cuda", generator=gen)
+    state /= torch.linalg.vector_norm(state)
+    state = state.to(cdtype)
+    for _ in range(warmup):
+        op.apply_in_place(state, norb, nelec)
+    torch.cuda.synchronize()
+    buf = (ctypes.c_double * 6)()
+    _lib.call("ffsim_amd_profile_begin")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.apply_in_place(state, norb, nelec)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.call("ffsim_amd_profile_end", buf)
+    fms, fn_, fb, dms, dn, db_ = list(buf)
+    del state
+    torch.cuda.empty_cache()
+    return {"apps_per_s": round(steps / elapsed, 3), "ms_per_app": round(1e3 * elapsed / steps, 2),
+            "fiber_avg_ms": round(fms / max(fn_, 1), 4),
+            "fiber_GBs": round(fb / max(fms, 1e-9) / 1e6, 1),
+            "fiber_launches_per_app": fn_ / steps, "diag_launches_per_app": dn / steps}
+
+
+def gates_leg(norb: int = 16, reps: int = 5, seed: int = 21) -> dict:
+    """The plain drop-ins at norb=16 (8,8) on a resident state, complex128 and
+    complex64: apply_orbital_rotation with a Haar 16x16 unitary (both sectors),
+    apply_diag_coulomb_evolution (random symmetric J, U(-1,1), t=1.0; the
+    separate diagonal kernel) and apply_num_op_sum_evolution (random 16-vector,
+    U(-1,1), t=1.0).  Per call: wall ms and kernel GB/s (algorithmic bytes over
+    the kernels' event time)."""
+    import torch
+
+    import ffsim_amd
+
+    nelec = (norb // 2, norb // 2)
+    rng = np.random.default_rng(seed)
+    U = ffsim_amd.random.random_unitary(norb, seed=rng)
+    vals = rng.uniform(-1, 1, size=norb * (norb + 1) // 2)
+    J = np.zeros((norb, norb))
+    J[np.triu_indices(norb)] = vals
+    J = J + np.triu(J, 1).T
+    lam = rng.uniform(-1, 1, size=norb)
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    out = {"workload": f"norb={norb} nelec={nelec} ({da}x{db}), Haar U, J U(-1,1) t=1.0, "
+                       f"num-op coefficients U(-1,1) t=1.0"}
+    for cdt, tag in ((torch.complex128, "c128"), (torch.complex64, "c64")):
+        psi = torch.randn(da * db, dtype=torch.complex128, device="cuda")
+        psi = (psi / torch.linalg.vector_norm(psi)).to(cdt)
+        out[tag] = {
+            "orbital_rotation": _profiled(
+                lambda: ffsim_amd.apply_orbital_rotation(psi, U, norb, nelec, copy=False), reps),
+            "diag_coulomb_evolution": _profiled(
+                lambda: ffsim_amd.apply_diag_coulomb_evolution(psi, J, 1.0, norb, nelec,
+                                                               copy=False), reps),
+            "num_op_sum_evolution": _profiled(
+                lambda: ffsim_amd.apply_num_op_sum_evolution(psi, lam, 1.0, norb, nelec,
+                                                             copy=False), reps),
+        }
+        log(f"gates {tag}: {out[tag]}")
+        del psi
+        torch.cuda.empty_cache()
+    return out
+
+
+def numpy_io_leg(op, norb: int, nelec, reps: int = 3, seed: int = 0) -> dict:
+    """The drop-in as ffsim's API takes it: numpy state in, numpy state out
+    (PCIe both ways through pinned bounce buffers), LUCJ n_reps=4, complex128."""
+    import ffsim_amd
+
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    vec = ffsim_amd.random.random_state_vector(da * db, seed=seed)
+    ffsim_amd.apply_unitary(vec, op, norb, nelec)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ffsim_amd.apply_unitary(vec, op, norb, nelec)
+    elapsed = (time.perf_counter() - t0) / reps
+    return {"workload": f"apply_unitary(numpy vector, LUCJ n_reps=4, norb={norb}, {nelec}), "
+                        f"complex128, numpy in and out",
+            "apps_per_s": round(1.0 / elapsed, 3), "ms_per_app": round(elapsed * 1e3, 1)}
 
 
 def trotter_leg(norb: int = 18, n_factors: int = 10, time_step: float = 0.1, seed: int = 5) -> dict:
@@ -432,9 +603,13 @@ def main() -> int:
                     help="skip the c64-vs-c128 comparison (keeps profiler runs to one dtype)")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other-dtype LUCJ rate, the plain gates and the numpy-io legs")
     ap.add_argument("--serving-requests", type=int, default=400)
     ap.add_argument("--serving-load", type=float, default=0.9)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads for the CPU baseline (0: every CPU this process may use, "
+                         "and nproc too when that is more)")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -561,9 +736,17 @@ def main() -> int:
         result["c64_vs_c128"] = c64_error(op, state, norb, nelec)
         log(f"c64 vs c128 (one application): {result['c64_vs_c128']}")
     vec_host = None if args.no_cpu_baseline else state.to(torch.complex128).cpu().numpy()
+    del state
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        other = torch.complex64 if args.dtype == "c128" else torch.complex128
+        tag = "c64" if args.dtype == "c128" else "c128"
+        result[f"lucj_{tag}"] = lucj_rate(op, norb, nelec, other, args.steps, args.warmup)
+        log(f"lucj {tag}: {result[f'lucj_{tag}']}")
+        result["gates_norb16"] = gates_leg(norb)
+        result["numpy_io"] = numpy_io_leg(op, norb, nelec)
+        log(f"numpy io: {result['numpy_io']}")
     if not args.no_trotter:
-        del state
-        torch.cuda.empty_cache()
         log("trotter leg")
         result["trotter_norb18"] = trotter_leg()
     if not args.no_gradient:
@@ -575,9 +758,17 @@ def main() -> int:
         result["serving"] = serving_leg(args.serving_requests, args.serving_load)
         log(f"serving: {result['serving']}")
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, args.cpu_threads)
+        info = host_cpu_info()
+        if args.cpu_threads > 0:
+            counts = [args.cpu_threads]
+        else:
+            counts = sorted({usable_cpus(info), int(info.get("nproc") or 1)})
+        log(f"cpu host: {info}; thread counts {counts}")
+        result["cpu_baseline"] = cpu_baseline(norb, nelec, op, vec_host, counts)
+        result["cpu_baseline"]["host"] = info
         if not args.no_gradient:
-            result["cpu_baseline"]["gradient_norb12"] = cpu_baseline_gradient(args.cpu_threads)
+            result["cpu_baseline"]["gradient_norb12"] = cpu_baseline_gradient(
+                result["cpu_baseline"]["cores"])
         result["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 2)
     print(json.dumps(result), flush=True)
     return 0

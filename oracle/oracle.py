@@ -40,20 +40,30 @@ import scipy.linalg
 
 _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "libffsim_oracle.so"
+# Built for the CPU it runs on (-march=native, as the reference's
+# .cargo/config.toml:2 builds ffsim); used when FFSIM_ORACLE_NATIVE=1.
+_NATIVE_LIB_PATH = _HERE / "libffsim_oracle_native.so"
+
+
+def _use_native() -> bool:
+    return os.environ.get("FFSIM_ORACLE_NATIVE") == "1"
 
 
 def build(force: bool = False) -> Path:
-    """Compile ffsim_oracle.c into libffsim_oracle.so (recipe: oracle/Makefile)."""
-    if force or not _LIB_PATH.exists():
-        subprocess.run(["make", "-C", str(_HERE)], check=True, capture_output=True)
-    return _LIB_PATH
+    """Compile ffsim_oracle.c (recipe: oracle/Makefile): the portable library,
+    or with FFSIM_ORACLE_NATIVE=1 the -march=native one for this host."""
+    path = _NATIVE_LIB_PATH if _use_native() else _LIB_PATH
+    if force or not path.exists():
+        target = ["native"] if _use_native() else []
+        subprocess.run(["make", "-B", "-C", str(_HERE), *target] if force else
+                       ["make", "-C", str(_HERE), *target], check=True, capture_output=True)
+    return path
 
 
 @functools.cache
 def _lib() -> ctypes.CDLL:
-    if not _LIB_PATH.exists():
-        build()
-    lib = ctypes.CDLL(str(_LIB_PATH))
+    path = build()
+    lib = ctypes.CDLL(str(path))
     P = ctypes.c_void_p
     L = ctypes.c_long
     D = ctypes.c_double

@@ -150,7 +150,7 @@ def check_optimize_kwargs(kwargs: dict, allowed=OPTIMIZE_ARGS) -> None:
 def _check_t2_options(max_terms, optimize: bool, kwargs: dict) -> None:
     check_optimize_kwargs(kwargs, _T2_ARGS)
     if max_terms is not None and max_terms < 1:
-        raise ValueError(f"max_terms must be at least 1. Got {max_terms}.")
+        raise ValueError(f"max_terms must be a positive count, got {max_terms}")
     if optimize:
         raise NotImplementedError(
             "optimize=True (compressed double factorization) is not supported; "
@@ -295,7 +295,7 @@ def double_factorized(two_body_tensor: np.ndarray, *, tol: float = 1e-8,
                                                             "return_optimize_result"})
     two_body_tensor = np.asarray(two_body_tensor)
     if np.iscomplexobj(two_body_tensor):
-        raise ValueError("Double factorization of complex two-body tensors is not supported.")
+        raise ValueError("the two-body tensor must be real: complex tensors cannot be double-factorized here")
     norb = two_body_tensor.shape[0]
     if not norb:
         return np.empty((0, 0, 0)), np.empty((0, 0, 0))

@@ -21,12 +21,8 @@ def apply_unitary(vec, obj: Any, norb: int, nelec, copy: bool = True):
         if result is not NotImplemented:
             return result
     raise TypeError(
-        "ffsim.apply_unitary failed. "
-        "Object doesn't have a unitary effect.\n"
-        f"type: {type(obj)}\n"
-        f"object: {obj!r}\n"
-        "The object did not have an _apply_unitary_ method that returned "
-        "a value besides NotImplemented."
+        f"apply_unitary: {type(obj)} has no _apply_unitary_, or it returned NotImplemented "
+        f"for norb={norb}, nelec={nelec} (object: {obj!r})"
     )
 
 
@@ -36,7 +32,7 @@ def linear_operator(obj: Any, norb: int, nelec):
     method = getattr(obj, "_linear_operator_", None)
     if method is not None:
         return method(norb=norb, nelec=nelec)
-    raise TypeError(f"Object of type {type(obj)} has no _linear_operator_ method.")
+    raise TypeError(f"linear_operator: {type(obj)} defines no _linear_operator_.")
 
 
 def approx_eq(obj: Any, other: Any, rtol: float = 1e-5, atol: float = 1e-8) -> bool:
@@ -62,8 +58,7 @@ def diag(obj: Any, norb: int, nelec):
     if method is not None:
         return method(norb=norb, nelec=nelec)
     raise TypeError(
-        f"Could not compute diagonal entries of object of type {type(obj)}.\n"
-        "The object did not have a _diag_ method that returned its diagonal entries."
+        f"diag: {type(obj)} defines no _diag_ that returns its diagonal."
     )
 
 
@@ -77,7 +72,6 @@ def trace(obj: Any, norb: int, nelec) -> float:
     if method is not None:
         return np.sum(method(norb=norb, nelec=nelec))
     raise TypeError(
-        f"Could not compute trace of object of type {type(obj)}.\n"
-        "The object did not have a _trace_ method that returned the trace, or "
-        "a _diag_ method that returned its diagonal entries."
+        f"trace: {type(obj)} defines neither _trace_ nor a _diag_ that returns its "
+        "diagonal."
     )

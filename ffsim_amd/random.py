@@ -17,7 +17,7 @@ from ffsim_amd.variational import UCJOpSpinBalanced
 def random_state_vector(dim: int, *, seed=None, dtype=complex) -> np.ndarray:
     """random.py:24-47: N(0,1) real parts, then imaginary parts, normalized."""
     if dim < 1:
-        raise ValueError("Dimension must be at least one.")
+        raise ValueError("the dimension must be positive")
     rng = np.random.default_rng(seed)
     vec = rng.standard_normal(dim).astype(dtype, copy=False)
     if np.issubdtype(dtype, np.complexfloating):

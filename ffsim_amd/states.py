@@ -52,8 +52,7 @@ def convert_bitstring_type(strings, input_type: BitstringType, output_type: Bits
     for t, name in ((input_type, "input_type"), (output_type, "output_type")):
         if not isinstance(t, BitstringType):
             raise TypeError(
-                f"{name} must be ffsim.BitstringType.STRING, ffsim.BitstringType.INT, "
-                f"or ffsim.BitstringType.BIT_ARRAY. Got {t}.")
+                f"{name}: expected a BitstringType (STRING, INT or BIT_ARRAY), got {t}.")
     if input_type is output_type:
         return strings
     if input_type is BitstringType.BIT_ARRAY:

@@ -84,9 +84,9 @@ def simulate_trotter_double_factorized(
     """Evolve by exp(-i H time) with the Trotter-Suzuki formula
     (python/ffsim/trotter/double_factorized.py:25-86)."""
     if order < 0:
-        raise ValueError(f"order must be non-negative, got {order}.")
+        raise ValueError(f"the Trotter order cannot be negative (got {order})")
     if n_steps < 0:
-        raise ValueError(f"n_steps must be non-negative, got {n_steps}.")
+        raise ValueError(f"the number of Trotter steps cannot be negative (got {n_steps})")
     st = _device.to_device(vec, copy)
     if n_steps == 0:
         return st.restore(st.tensor)
@@ -173,9 +173,9 @@ def simulate_trotter_diag_coulomb_split_op(
     (python/ffsim/trotter/diagonal_coulomb_split_op.py:25-84), as one gate
     sequence on the GPU."""
     if order < 0:
-        raise ValueError(f"order must be non-negative, got {order}.")
+        raise ValueError(f"the Trotter order cannot be negative (got {order})")
     if n_steps < 0:
-        raise ValueError(f"n_steps must be non-negative, got {n_steps}.")
+        raise ValueError(f"the number of Trotter steps cannot be negative (got {n_steps})")
     st = _device.to_device(vec, copy)
     if n_steps == 0:
         return st.restore(st.tensor)

@@ -33,18 +33,18 @@ class GivensAnsatzOp:
         """Validation of givens.py:54-69."""
         if len(self.thetas) != len(self.interaction_pairs):
             raise ValueError(
-                "The number of thetas must equal the number of interaction pairs. "
-                f"Got {len(self.thetas)} and {len(self.interaction_pairs)}."
+                f"{len(self.thetas)} thetas for {len(self.interaction_pairs)} interaction pairs; "
+                "there must be one per pair"
             )
         if self.phis is not None and len(self.phis) != len(self.interaction_pairs):
             raise ValueError(
-                "The number of phis must equal the number of interaction pairs. "
-                f"Got {len(self.phis)} and {len(self.interaction_pairs)}."
+                f"{len(self.phis)} phis for {len(self.interaction_pairs)} interaction pairs; "
+                "there must be one per pair"
             )
         if self.phase_angles is not None and len(self.phase_angles) != self.norb:
             raise ValueError(
-                "The number of phase angles must equal the number of orbitals. "
-                f"Got {len(self.phase_angles)} and {self.norb}."
+                f"{len(self.phase_angles)} phase angles for {self.norb} orbitals; "
+                "there must be one per orbital"
             )
 
     def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
@@ -73,9 +73,7 @@ class GivensAnsatzOp:
         n_params = GivensAnsatzOp.n_params(norb, interaction_pairs, with_phis, with_phase_angles)
         if len(params) != n_params:
             raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
+                f"these settings take {n_params} parameters, got {len(params)}"
             )
         params = np.asarray(params, dtype=float)
         thetas = params[:n_pairs]

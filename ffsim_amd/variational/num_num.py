@@ -1,13 +1,20 @@
-"""Number-number interaction ansatz (python/ffsim/variational/num_num.py).
+"""Spin-balanced number-number interaction ansatz.
 
-Angles on same-spin and opposite-spin orbital pairs; as an operator it is the
-diagonal Coulomb evolution exp(i J) with J built from the angles, applied by
-the GPU diagonal kernel (num_num.py:70-85).
+Public layout (python/ffsim/variational/num_num.py): the operator holds, for
+same-spin and opposite-spin interactions, a list of upper-triangular orbital
+pairs and one angle per pair; its parameter vector is the same-spin angles
+followed by the opposite-spin angles, and as a unitary it is the diagonal
+Coulomb evolution exp(i sum_pairs theta n n) (time -1 with J = the angles).
+
+Here the pairs are kept as index arrays, so the matrices are built by one
+scatter per spin block and ``from_diag_coulomb_mats`` reads the nonzero
+entries of the upper triangle (row by row, diagonal included: the order of
+``np.triu_indices``) in one vectorized pass.  Applying the operator is one
+call of the GPU diagonal kernel.
 """
 
 from __future__ import annotations
 
-import itertools
 from dataclasses import dataclass
 
 import numpy as np
@@ -15,37 +22,101 @@ import numpy as np
 from ffsim_amd.gates.diag_coulomb import apply_diag_coulomb_evolution
 
 
-def _validate_pairs(pairs) -> None:
+def _pair_array(pairs) -> np.ndarray:
+    arr = np.asarray(list(pairs) if pairs is not None else [], dtype=np.int64)
+    return arr.reshape(-1, 2)
+
+
+def _check_pairs(pairs, block: str) -> None:
+    """Pairs must be distinct and upper triangular (i <= j); ValueError otherwise."""
     if pairs is None:
         return
-    if len(set(pairs)) != len(pairs):
-        raise ValueError(f"Duplicate interaction pairs encountered: {pairs}.")
-    for i, j in pairs:
-        if i > j:
-            raise ValueError(
-                "You must provide only upper triangular interaction pairs. "
-                f"Got {(i, j)}, which is a lower triangular pair."
-            )
+    arr = _pair_array(pairs)
+    if len({(int(i), int(j)) for i, j in arr}) != len(arr):
+        raise ValueError(f"{block} interaction pairs contain a repeated pair: {list(pairs)}.")
+    below = arr[arr[:, 0] > arr[:, 1]]
+    if len(below):
+        i, j = (int(x) for x in below[0])
+        raise ValueError(
+            f"{block} interaction pair {(i, j)} is lower triangular; give each pair as "
+            f"(i, j) with i <= j."
+        )
 
 
 @dataclass(frozen=True)
 class NumNumAnsatzOpSpinBalanced:
-    """Number-number interactions with angles on (pairs_aa, pairs_ab)."""
+    """Number-number interactions with angles ``thetas`` on ``interaction_pairs``.
+
+    ``interaction_pairs`` is ``(pairs_aa, pairs_ab)`` and ``thetas`` the matching
+    ``(angles_aa, angles_ab)``.
+    """
 
     norb: int
     interaction_pairs: tuple
     thetas: tuple
 
     def __post_init__(self):
-        """Validation of num_num.py:60-68."""
-        for pairs, angles in zip(self.interaction_pairs, self.thetas):
-            _validate_pairs(pairs)
-            if len(pairs) != len(angles):
+        for block, pairs, angles in zip(("same-spin", "opposite-spin"), self.interaction_pairs,
+                                        self.thetas):
+            _check_pairs(pairs, block)
+            n_pairs, n_angles = len(_pair_array(pairs)), len(np.atleast_1d(angles))
+            if n_pairs != n_angles:
                 raise ValueError(
-                    "The number of interaction pairs must be equal to the number of "
-                    f"rotation angles.Got {len(pairs)}, and {len(angles)}."
+                    f"{block} block: {n_pairs} interaction pairs but {n_angles} angles; "
+                    f"they must match."
                 )
 
+    # -- parameters --------------------------------------------------------
+    @staticmethod
+    def n_params(interaction_pairs) -> int:
+        return int(sum(len(_pair_array(p)) for p in interaction_pairs))
+
+    def to_parameters(self) -> np.ndarray:
+        return np.concatenate([np.asarray(t, dtype=float).reshape(-1) for t in self.thetas])
+
+    @staticmethod
+    def from_parameters(params: np.ndarray, norb: int,
+                        interaction_pairs) -> "NumNumAnsatzOpSpinBalanced":
+        params = np.asarray(params, dtype=float).reshape(-1)
+        n_aa = len(_pair_array(interaction_pairs[0]))
+        expected = NumNumAnsatzOpSpinBalanced.n_params(interaction_pairs)
+        if params.size != expected:
+            raise ValueError(
+                f"these interaction pairs take {expected} parameters, got {params.size}."
+            )
+        return NumNumAnsatzOpSpinBalanced(
+            norb=norb, interaction_pairs=interaction_pairs,
+            thetas=(params[:n_aa].copy(), params[n_aa:].copy()),
+        )
+
+    # -- matrices ----------------------------------------------------------
+    def to_diag_coulomb_mats(self) -> np.ndarray:
+        """(2, norb, norb): the symmetric same-spin and opposite-spin angle matrices."""
+        mats = np.zeros((2, self.norb, self.norb))
+        for mat, pairs, angles in zip(mats, self.interaction_pairs, self.thetas):
+            idx = _pair_array(pairs)
+            if len(idx):
+                vals = np.asarray(angles, dtype=float).reshape(-1)
+                mat[idx[:, 0], idx[:, 1]] = vals
+                mat[idx[:, 1], idx[:, 0]] = vals
+        return mats
+
+    @staticmethod
+    def from_diag_coulomb_mats(diag_coulomb_mats) -> "NumNumAnsatzOpSpinBalanced":
+        """The nonzero upper-triangle entries of each matrix, row by row."""
+        mat_aa, mat_ab = (np.asarray(m) for m in diag_coulomb_mats)
+        norb = mat_aa.shape[0]
+        rows, cols = np.triu_indices(norb)
+        pairs, angles = [], []
+        for mat in (mat_aa, mat_ab):
+            vals = mat[rows, cols]
+            keep = vals != 0
+            pairs.append([(int(i), int(j)) for i, j in zip(rows[keep], cols[keep])])
+            angles.append(np.array(vals[keep], dtype=float))
+        return NumNumAnsatzOpSpinBalanced(norb=norb, interaction_pairs=tuple(pairs),
+                                          thetas=tuple(angles))
+
+    # -- protocols ---------------------------------------------------------
     def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
         if isinstance(nelec, int):
             return NotImplemented
@@ -53,60 +124,14 @@ class NumNumAnsatzOpSpinBalanced:
         return apply_diag_coulomb_evolution(vec, (mat_aa, mat_ab, mat_aa), -1.0, norb, nelec,
                                             copy=copy)
 
-    @staticmethod
-    def n_params(interaction_pairs) -> int:
-        return sum(len(pairs) for pairs in interaction_pairs)
-
-    def to_parameters(self) -> np.ndarray:
-        return np.concatenate([np.asarray(t, dtype=float) for t in self.thetas])
-
-    @staticmethod
-    def from_parameters(params: np.ndarray, norb: int, interaction_pairs) -> "NumNumAnsatzOpSpinBalanced":
-        """num_num.py:103-131."""
-        pairs_aa, pairs_ab = interaction_pairs
-        n_params = len(pairs_aa) + len(pairs_ab)
-        if len(params) != n_params:
-            raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
-            )
-        params = np.asarray(params, dtype=float)
-        return NumNumAnsatzOpSpinBalanced(
-            norb=norb, interaction_pairs=interaction_pairs,
-            thetas=(params[: len(pairs_aa)], params[len(pairs_aa) :]),
-        )
-
-    @staticmethod
-    def from_diag_coulomb_mats(diag_coulomb_mats) -> "NumNumAnsatzOpSpinBalanced":
-        """Nonzero upper-triangle entries of (J_aa, J_ab) (num_num.py:133-163)."""
-        mat_aa, mat_ab = diag_coulomb_mats
-        norb = mat_aa.shape[0]
-        pairs = ([], [])
-        thetas = ([], [])
-        for mat, ps, ts in zip((mat_aa, mat_ab), pairs, thetas):
-            for pair in itertools.combinations_with_replacement(range(norb), 2):
-                if mat[pair]:
-                    ps.append(pair)
-                    ts.append(mat[pair])
-        return NumNumAnsatzOpSpinBalanced(norb=norb, interaction_pairs=pairs,
-                                          thetas=(np.array(thetas[0]), np.array(thetas[1])))
-
-    def to_diag_coulomb_mats(self) -> np.ndarray:
-        """(2, norb, norb): symmetric J_aa and J_ab from the angles."""
-        mats = np.zeros((2, self.norb, self.norb))
-        for mat, pairs, angles in zip(mats, self.interaction_pairs, self.thetas):
-            for (i, j), theta in zip(pairs, angles):
-                mat[i, j] = theta
-                mat[j, i] = theta
-        return mats
-
     def _approx_eq_(self, other, rtol: float, atol: float) -> bool:
-        if isinstance(other, NumNumAnsatzOpSpinBalanced):
-            if self.norb != other.norb:
+        if not isinstance(other, NumNumAnsatzOpSpinBalanced):
+            return NotImplemented
+        if self.norb != other.norb:
+            return False
+        for p, q in zip(self.interaction_pairs, other.interaction_pairs):
+            if not np.array_equal(_pair_array(p), _pair_array(q)):
                 return False
-            if [list(p) for p in self.interaction_pairs] != [list(p) for p in other.interaction_pairs]:
-                return False
-            return all(np.allclose(a, b, rtol=rtol, atol=atol)
-                       for a, b in zip(self.thetas, other.thetas))
-        return NotImplemented
+        return all(np.allclose(np.asarray(a, dtype=float), np.asarray(b, dtype=float),
+                               rtol=rtol, atol=atol)
+                   for a, b in zip(self.thetas, other.thetas))

@@ -32,9 +32,8 @@ class UCJAnglesOpSpinBalanced:
     def __post_init__(self):
         if len(self.num_num_ansatz_ops) != len(self.givens_ansatz_ops):
             raise ValueError(
-                "The number of number-number ansatz operations must equal the number "
-                "of Givens ansatz operations. "
-                f"Got {len(self.num_num_ansatz_ops)} and {len(self.givens_ansatz_ops)}."
+                f"{len(self.num_num_ansatz_ops)} number-number layers and "
+                f"{len(self.givens_ansatz_ops)} Givens layers; each repetition needs one of each"
             )
 
     @property
@@ -62,9 +61,7 @@ class UCJAnglesOpSpinBalanced:
         )
         if len(params) != n_params:
             raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
+                f"these settings take {n_params} parameters, got {len(params)}"
             )
         n_nn = sum(len(p) for p in num_num_interaction_pairs)
         n_gv = 2 * len(givens_interaction_pairs) + norb

@@ -23,14 +23,13 @@ def _validate_interaction_pairs(pairs, ordered: bool) -> None:
     if pairs is None:
         return
     if len(set(pairs)) != len(pairs):
-        raise ValueError(f"Duplicate interaction pairs encountered: {pairs}.")
+        raise ValueError(f"interaction pairs contain a repeated pair: {pairs}.")
     if not ordered:
         for i, j in pairs:
             if i > j:
                 raise ValueError(
-                    "When specifying spinless, alpha-alpha or beta-beta interaction pairs, "
-                    f"you must provide only upper triangular pairs. Got {(i, j)}, which is a "
-                    "lower triangular pair."
+                    f"same-spin (or spinless) interaction pair {(i, j)} is lower triangular; "
+                    "give each pair as (i, j) with i <= j"
                 )
 
 
@@ -51,36 +50,36 @@ class UCJOpSpinBalanced:
             return
         if self.diag_coulomb_mats.ndim != 4 or self.diag_coulomb_mats.shape[1] != 2:
             raise ValueError(
-                "diag_coulomb_mats should have shape (n_reps, 2, norb, norb). "
-                f"Got shape {self.diag_coulomb_mats.shape}."
+                "expected diag_coulomb_mats of shape (n_reps, 2, norb, norb); "
+                f"got {self.diag_coulomb_mats.shape}"
             )
         if self.orbital_rotations.ndim != 3:
             raise ValueError(
-                "orbital_rotations should have shape (n_reps, norb, norb). "
-                f"Got shape {self.orbital_rotations.shape}."
+                "expected orbital_rotations of shape (n_reps, norb, norb); "
+                f"got {self.orbital_rotations.shape}"
             )
         if self.final_orbital_rotation is not None and self.final_orbital_rotation.ndim != 2:
             raise ValueError(
-                "final_orbital_rotation should have shape (norb, norb). "
-                f"Got shape {self.final_orbital_rotation.shape}."
+                "expected final_orbital_rotation of shape (norb, norb); "
+                f"got {self.final_orbital_rotation.shape}"
             )
         if self.diag_coulomb_mats.shape[0] != self.orbital_rotations.shape[0]:
             raise ValueError(
-                "diag_coulomb_mats and orbital_rotations should have the same first dimension. "
-                f"Got {self.diag_coulomb_mats.shape[0]} and {self.orbital_rotations.shape[0]}."
+                f"diag_coulomb_mats and orbital_rotations hold {self.diag_coulomb_mats.shape[0]} and "
+                f"{self.orbital_rotations.shape[0]} repetitions; they must match"
             )
         if not all(
             _linalg.is_real_symmetric(m[0], rtol=rtol, atol=atol)
             and _linalg.is_real_symmetric(m[1], rtol=rtol, atol=atol)
             for m in self.diag_coulomb_mats
         ):
-            raise ValueError("Diagonal Coulomb matrices were not all real symmetric.")
+            raise ValueError("every diagonal Coulomb matrix must be real symmetric")
         if not all(_linalg.is_unitary(u, rtol=rtol, atol=atol) for u in self.orbital_rotations):
-            raise ValueError("Orbital rotations were not all unitary.")
+            raise ValueError("every orbital rotation must be unitary")
         if self.final_orbital_rotation is not None and not _linalg.is_unitary(
             self.final_orbital_rotation, rtol=rtol, atol=atol
         ):
-            raise ValueError("Final orbital rotation was not unitary.")
+            raise ValueError("the final orbital rotation must be unitary")
 
     @property
     def norb(self) -> int:
@@ -112,9 +111,7 @@ class UCJOpSpinBalanced:
         )
         if len(params) != n_params:
             raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
+                f"these settings take {n_params} parameters, got {len(params)}"
             )
         pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
         triu = list(itertools.combinations_with_replacement(range(norb), 2))
@@ -162,7 +159,7 @@ class UCJOpSpinBalanced:
 
         _linalg.check_optimize_kwargs(optimize_kwargs)
         if isinstance(n_reps, int) and n_reps <= 0:
-            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+            raise ValueError(f"n_reps must be positive, got {n_reps}")
         pairs_aa, pairs_ab = interaction_pairs if interaction_pairs is not None else (None, None)
         _validate_interaction_pairs(pairs_aa, ordered=False)
         _validate_interaction_pairs(pairs_ab, ordered=False)
@@ -203,8 +200,8 @@ class UCJOpSpinBalanced:
         c0 = vec[0]
         if abs(c0) <= c0_threshold:
             raise ValueError(
-                f"CISD reference coefficient c0={c0} is smaller than the specified "
-                f"threshold, c0_tol={c0_threshold}.")
+                f"CISD vector's reference amplitude c0={c0} is within the threshold "
+                f"c0_tol={c0_threshold} of zero")
         t1 = vec[1 : 1 + n1].reshape(nocc, nvrt) / c0
         t2 = vec[1 + n1 :].reshape(nocc, nocc, nvrt, nvrt) / c0
         t2 = t2 - 0.5 * t1[:, None, :, None] * t1[None, :, None, :]

@@ -38,24 +38,23 @@ class UCJOpSpinUnbalanced:
             return
         if self.diag_coulomb_mats.ndim != 4 or self.diag_coulomb_mats.shape[1] != 3:
             raise ValueError(
-                "diag_coulomb_mats should have shape (n_reps, 3, norb, norb). "
-                f"Got shape {self.diag_coulomb_mats.shape}."
+                "expected diag_coulomb_mats of shape (n_reps, 3, norb, norb); "
+                f"got {self.diag_coulomb_mats.shape}"
             )
         if self.orbital_rotations.ndim != 4 or self.orbital_rotations.shape[1] != 2:
             raise ValueError(
-                "orbital_rotations should have shape (n_reps, 2, norb, norb). "
-                f"Got shape {self.orbital_rotations.shape}."
+                "expected orbital_rotations of shape (n_reps, 2, norb, norb); "
+                f"got {self.orbital_rotations.shape}"
             )
         if self.final_orbital_rotation is not None and self.final_orbital_rotation.ndim != 3:
             raise ValueError(
-                "final_orbital_rotation should have shape (2, norb, norb). "
-                f"Got shape {self.final_orbital_rotation.shape}."
+                "expected final_orbital_rotation of shape (2, norb, norb); "
+                f"got {self.final_orbital_rotation.shape}"
             )
         if self.diag_coulomb_mats.shape[0] != self.orbital_rotations.shape[0]:
             raise ValueError(
-                "diag_coulomb_mats and orbital_rotations should have the same first "
-                f"dimension. Got {self.diag_coulomb_mats.shape[0]} and "
-                f"{self.orbital_rotations.shape[0]}."
+                f"diag_coulomb_mats and orbital_rotations hold {self.diag_coulomb_mats.shape[0]} and "
+                f"{self.orbital_rotations.shape[0]} repetitions; they must match"
             )
         if not all(
             _linalg.is_real_symmetric(m[0], rtol=rtol, atol=atol)
@@ -63,20 +62,19 @@ class UCJOpSpinUnbalanced:
             for m in self.diag_coulomb_mats
         ):
             raise ValueError(
-                "alpha-alpha and beta-beta diagonal Coulomb matrices were not all "
-                "real symmetric."
+                "the same-spin (aa and bb) diagonal Coulomb matrices must be real symmetric"
             )
         if not all(
             _linalg.is_unitary(u[0], rtol=rtol, atol=atol)
             and _linalg.is_unitary(u[1], rtol=rtol, atol=atol)
             for u in self.orbital_rotations
         ):
-            raise ValueError("Orbital rotations were not all unitary.")
+            raise ValueError("every orbital rotation must be unitary")
         if self.final_orbital_rotation is not None and not (
             _linalg.is_unitary(self.final_orbital_rotation[0], rtol=rtol, atol=atol)
             and _linalg.is_unitary(self.final_orbital_rotation[1], rtol=rtol, atol=atol)
         ):
-            raise ValueError("Final orbital rotation was not unitary.")
+            raise ValueError("the final orbital rotation must be unitary")
 
     @property
     def norb(self) -> int:
@@ -123,9 +121,7 @@ class UCJOpSpinUnbalanced:
         )
         if len(params) != n_params:
             raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
+                f"these settings take {n_params} parameters, got {len(params)}"
             )
         pairs_aa, pairs_ab, pairs_bb = UCJOpSpinUnbalanced._pairs(norb, interaction_pairs)
         mats = np.zeros((n_reps, 3, norb, norb))
@@ -177,7 +173,7 @@ class UCJOpSpinUnbalanced:
 
         _linalg.check_optimize_kwargs(optimize_kwargs)
         if isinstance(n_reps, int) and n_reps <= 0:
-            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+            raise ValueError(f"n_reps must be positive, got {n_reps}")
         pairs_aa, pairs_ab, pairs_bb = (
             interaction_pairs if interaction_pairs is not None else (None, None, None)
         )
@@ -203,7 +199,7 @@ class UCJOpSpinUnbalanced:
             mats_ab, rots_ab = empty()
         else:
             raise ValueError(
-                f"Number of alpha-beta repetitions must be at least 1. Got {cap_ab}.")
+                f"the alpha-beta (opposite-spin) part needs a positive number of repetitions, got {cap_ab}")
 
         if n_reps is None:
             cap_same = None
@@ -230,7 +226,7 @@ class UCJOpSpinUnbalanced:
             mats_same, rots_same = empty()
         else:
             raise ValueError(
-                f"Number of alpha-alpha repetitions must be at least 1. Got {cap_same}.")
+                f"the same-spin part needs a positive number of repetitions, got {cap_same}")
 
         if n_reps is None or isinstance(n_reps, int):
             mats, rots = util.pad_reps(np.concatenate([mats_ab, mats_same])[:n_reps],

@@ -37,34 +37,33 @@ class UCJOpSpinless:
             return
         if self.diag_coulomb_mats.ndim != 3:
             raise ValueError(
-                "diag_coulomb_mats should have shape (n_reps, norb, norb). "
-                f"Got shape {self.diag_coulomb_mats.shape}."
+                "expected diag_coulomb_mats of shape (n_reps, norb, norb); "
+                f"got {self.diag_coulomb_mats.shape}"
             )
         if self.orbital_rotations.ndim != 3:
             raise ValueError(
-                "orbital_rotations should have shape (n_reps, norb, norb). "
-                f"Got shape {self.orbital_rotations.shape}."
+                "expected orbital_rotations of shape (n_reps, norb, norb); "
+                f"got {self.orbital_rotations.shape}"
             )
         if self.final_orbital_rotation is not None and self.final_orbital_rotation.ndim != 2:
             raise ValueError(
-                "final_orbital_rotation should have shape (norb, norb). "
-                f"Got shape {self.final_orbital_rotation.shape}."
+                "expected final_orbital_rotation of shape (norb, norb); "
+                f"got {self.final_orbital_rotation.shape}"
             )
         if self.diag_coulomb_mats.shape[0] != self.orbital_rotations.shape[0]:
             raise ValueError(
-                "diag_coulomb_mats and orbital_rotations should have the same first "
-                f"dimension. Got {self.diag_coulomb_mats.shape[0]} and "
-                f"{self.orbital_rotations.shape[0]}."
+                f"diag_coulomb_mats and orbital_rotations hold {self.diag_coulomb_mats.shape[0]} and "
+                f"{self.orbital_rotations.shape[0]} repetitions; they must match"
             )
         if not all(_linalg.is_real_symmetric(m, rtol=rtol, atol=atol)
                    for m in self.diag_coulomb_mats):
-            raise ValueError("Diagonal Coulomb matrices were not all real symmetric.")
+            raise ValueError("every diagonal Coulomb matrix must be real symmetric")
         if not all(_linalg.is_unitary(u, rtol=rtol, atol=atol) for u in self.orbital_rotations):
-            raise ValueError("Orbital rotations were not all unitary.")
+            raise ValueError("every orbital rotation must be unitary")
         if self.final_orbital_rotation is not None and not _linalg.is_unitary(
             self.final_orbital_rotation, rtol=rtol, atol=atol
         ):
-            raise ValueError("Final orbital rotation was not unitary.")
+            raise ValueError("the final orbital rotation must be unitary")
 
     @property
     def norb(self) -> int:
@@ -92,9 +91,7 @@ class UCJOpSpinless:
         )
         if len(params) != n_params:
             raise ValueError(
-                "The number of parameters passed did not match the number expected "
-                "based on the function inputs. "
-                f"Expected {n_params} but got {len(params)}."
+                f"these settings take {n_params} parameters, got {len(params)}"
             )
         pairs = (list(itertools.combinations_with_replacement(range(norb), 2))
                  if interaction_pairs is None else list(interaction_pairs))
@@ -130,7 +127,7 @@ class UCJOpSpinless:
 
         _linalg.check_optimize_kwargs(optimize_kwargs)
         if isinstance(n_reps, int) and n_reps <= 0:
-            raise ValueError(f"n_reps must be at least 1. Got {n_reps}.")
+            raise ValueError(f"n_reps must be positive, got {n_reps}")
         _validate_interaction_pairs(interaction_pairs, ordered=False)
         mats, rots = _linalg.double_factorized_t2(t2, tol=tol, max_terms=n_reps,
                                                   optimize=optimize, **optimize_kwargs)

@@ -36,3 +36,24 @@ class OracleAdjointOps:
 
     def host_phase(self, x):
         return x
+
+
+def five_point_derivative(func, theta, direction, h=1e-3):
+    """d/de func(theta + e * direction) at e = 0 by the five-point central stencil,
+    (-f(2h) + 8 f(h) - 8 f(-h) + f(-2h)) / (12 h), truncation O(h^4)."""
+    theta = np.asarray(theta, dtype=float)
+    d = np.asarray(direction, dtype=float)
+    f = lambda s: func(theta + s * d)  # noqa: E731
+    return (-f(2 * h) + 8 * f(h) - 8 * f(-h) + f(-2 * h)) / (12 * h)
+
+
+def five_point_gradient(func, theta, h=1e-3, indices=None):
+    """Five-point central finite differences along the coordinate axes."""
+    theta = np.asarray(theta, dtype=float)
+    indices = range(len(theta)) if indices is None else indices
+    out = []
+    for i in indices:
+        e = np.zeros_like(theta)
+        e[i] = 1.0
+        out.append(five_point_derivative(func, theta, e, h))
+    return np.array(out)

@@ -97,9 +97,11 @@ def test_autograd_matches_oracle_finite_difference(rng):
 
     e_ref = O.ucj_energy(theta, h, psi0, norb, nelec, n_reps, pairs, True)
     assert abs(energy.item() - e_ref) < 1e-10
-    fd = O.fd_gradient(lambda p: O.ucj_energy(p, h, psi0, norb, nelec, n_reps, pairs, True),
-                       theta, 1e-6)
-    np.testing.assert_allclose(params.grad.numpy(), fd, atol=1e-7, rtol=0)
+    from oracle_adjoint import five_point_derivative, five_point_gradient
+
+    fd = five_point_gradient(lambda p: O.ucj_energy(p, h, psi0, norb, nelec, n_reps, pairs, True),
+                             theta)
+    np.testing.assert_allclose(params.grad.numpy(), fd, atol=1e-9, rtol=0)
     # reference-state gradient: directional derivative along a random direction
     d = O.random_state_vector(psi0.size, rng)
     g_ref = ref.grad.cpu().numpy()
@@ -109,16 +111,16 @@ def test_autograd_matches_oracle_finite_difference(rng):
         w = O.apply_ucj_spin_balanced(v, J, U, F, norb, nelec)
         return np.vdot(w, h(w)).real
 
-    eps = 1e-6
-    fd_dir = (e_of(psi0 + eps * d) - e_of(psi0 - eps * d)) / (2 * eps)
-    assert abs(np.vdot(g_ref, d).real - fd_dir) < 1e-7
+    fd_dir = five_point_derivative(lambda s: e_of(psi0 + s[0] * d), np.zeros(1), np.ones(1))
+    assert abs(np.vdot(g_ref, d).real - fd_dir) < 1e-9
 
 
 def test_config_norb12_gradient_668_params(rng):
     """norb=12 (6,6), LUCJ n_reps=4 (668 parameters, U(-pi, pi), seed 12), Hartree-Fock
-    reference, synthetic H_DF with 8 factors (seed 7): adjoint vs GPU central FD on every
-    parameter (h=1e-6), vs the oracle's central FD on a subset, and vs the sweep over the
-    oracle's primitive steps."""
+    reference, synthetic H_DF with 8 factors (seed 7): adjoint vs five-point FD of the GPU
+    energy on every parameter, vs five-point FD of the oracle's energy on a subset and
+    along a random direction (all to 1e-9), and vs the sweep over the oracle's
+    primitive steps (1e-10)."""
     torch = _torch()
     import ffsim_amd
     from ffsim_amd.variational.gradient import UCJLayout, ucj_adjoint_sweep, ucj_energy_and_gradient
@@ -145,28 +147,37 @@ def test_config_norb12_gradient_668_params(rng):
         _apply_forward(layout, p, psi, nelec)
         return float(torch.vdot(psi, linop.matvec_device(psi)).real.item())
 
-    _case("config norb=12 GPU central FD, 1336 energies")
+    from oracle_adjoint import five_point_derivative, five_point_gradient
+
+    # Five-point central differences (h = 1e-3, truncation O(h^4)) reach about 1e-12 on
+    # these energies (checked against the exact sweep over the oracle's steps on the
+    # CPU), so the adjoint is held to 1e-9 absolute: a miss of 1e-7 fails.
+    fd_tol = 1e-9
+    _case("config norb=12 GPU five-point FD, 2672 energies")
     assert abs(gpu_energy(theta) - energy) < 1e-12
-    fd_gpu = O.fd_gradient(gpu_energy, theta, 1e-6)
-    # central-FD error at h=1e-6: truncation ~h^2 (third derivative)/6 ~ 1e-12, rounding
-    # ~|E| * 1e-13 / h, the energies carrying ~1e-13 relative rounding from thousands of
-    # Givens updates; the adjoint itself matches the oracle's sweep to 1e-10 below
-    fd_tol = 5e-7 * max(1.0, abs(energy))
-    print(f"E={energy:.12f} max|adjoint-FD|={np.abs(grad - fd_gpu).max():.2e} tol={fd_tol:.1e}",
-          flush=True)
+    fd_gpu = five_point_gradient(gpu_energy, theta)
+    print(f"E={energy:.12f} max|adjoint-FD(GPU)|={np.abs(grad - fd_gpu).max():.2e}", flush=True)
     np.testing.assert_allclose(grad, fd_gpu, atol=fd_tol, rtol=0)
 
     def h(v):
         return O.df_hamiltonian_matvec(v, df.one_body_tensor, df.diag_coulomb_mats,
                                        df.orbital_rotations, df.constant, norb, nelec, False)
 
-    _case("config norb=12 oracle central FD subset")
-    subset = [0, 5, 70, 143, 144, 150, 155, 160, 334, 500, 660, 667]
-    fd_cpu = O.fd_gradient(
-        lambda p: O.ucj_energy(p, h, psi0, norb, nelec, n_reps, pairs), theta, 1e-6, subset
-    )
+    def e_cpu_of(p):
+        return O.ucj_energy(p, h, psi0, norb, nelec, n_reps, pairs)
+
+    _case("config norb=12 oracle five-point FD on 6 parameters and one random direction")
+    subset = [0, 70, 144, 155, 334, 667]
+    fd_cpu = five_point_gradient(e_cpu_of, theta, indices=subset)
+    print(f"max|adjoint-FD(oracle)| on {subset}: {np.abs(grad[subset] - fd_cpu).max():.2e}",
+          flush=True)
     np.testing.assert_allclose(grad[subset], fd_cpu, atol=fd_tol, rtol=0)
-    e_cpu = O.ucj_energy(theta, h, psi0, norb, nelec, n_reps, pairs)
+    direction = np.random.default_rng(3).standard_normal(len(theta))
+    direction /= np.linalg.norm(direction)
+    d_cpu = five_point_derivative(e_cpu_of, theta, direction)
+    print(f"directional: adjoint {grad @ direction:.15f} oracle FD {d_cpu:.15f}", flush=True)
+    assert abs(grad @ direction - d_cpu) < fd_tol
+    e_cpu = e_cpu_of(theta)
     assert abs(e_cpu - energy) < 1e-10
 
     _case("config norb=12 sweep over oracle steps")

@@ -40,6 +40,9 @@ def build(verbose: bool = False) -> Path:
         "-Wall",
         "-Wno-unused-function",
         *[str(CSRC / s) for s in SOURCES],
+        "-L/opt/rocm/lib",
+        "-lrocblas",
+        "-Wl,-rpath,/opt/rocm/lib",
         "-o",
         str(tmp),
     ]

@@ -8,6 +8,7 @@
 // stores the transposed state into the caller's work buffer, the beta passes
 // run in MODE_ROWS there, and the last beta pass stores back transposed.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -191,6 +192,10 @@ struct ProgramRing {
 };
 
 struct Context {
+  // rocBLAS handle of the batched small-state path (created on first use;
+  // calls that use it hold blas_mu, the handle being bound to one stream at a time)
+  rocblas_handle blas = nullptr;
+  std::mutex blas_mu;
   int device = -1;
   size_t lds_max = 65536;
   int n_cu = 0;
@@ -417,7 +422,7 @@ struct Program {
   std::vector<char> host;      // parameter bytes
   size_t scratch_bytes = 0;    // device-only scratch, placed after the parameters
   size_t scratch_base = 0;     // device offset of the scratch (known at commit)
-  bool committed = false, finished = false;
+  bool prepared = false, committed = false, finished = false;
 
   Program(Context& c, hipStream_t s) : ctx(&c), stream(s) {
     ProgramRing& R = c.ring;
@@ -463,6 +468,7 @@ struct Program {
   }
   // Appends n bytes (16-byte aligned; zeros when src is null), returns the offset.
   size_t append(const void* src, size_t n) {
+    require(!prepared, FFSIM_ERR_VALUE, "program appended after its layout was fixed");
     const size_t off = (host.size() + 15) & ~size_t(15);
     require(off + n + scratch_bytes <= ProgramRing::kMaxBytes, FFSIM_ERR_VALUE,
             "gate program too large");
@@ -473,6 +479,7 @@ struct Program {
   // Reserves n bytes of device-only scratch; false when the call would exceed
   // the slot limit.  Addresses via scratch<T>(off) after commit().
   bool reserve(size_t n, size_t* off) {
+    require(!prepared, FFSIM_ERR_VALUE, "program scratch reserved after its layout was fixed");
     const size_t o = (scratch_bytes + 255) & ~size_t(255);
     if (o + n + host.size() + 256 > ProgramRing::kMaxBytes) return false;
     scratch_bytes = o + n;
@@ -487,7 +494,10 @@ struct Program {
   T* scratch(size_t off) const {
     return reinterpret_cast<T*>(ctx->ring.d[slot] + scratch_base + off);
   }
-  void commit() {
+  // Fixes the layout and the slot's buffers: device addresses (dev, scratch)
+  // are valid from here on; nothing may be appended or reserved afterwards.
+  void prepare() {
+    if (prepared) return;
     ProgramRing& R = ctx->ring;
     const size_t used = host.size();
     scratch_base = (used + 255) & ~size_t(255);
@@ -509,6 +519,17 @@ struct Program {
       HIP_CHECK(hipMalloc((void**)&R.d[slot], cap));
       R.cap_d[slot] = cap;
     }
+    prepared = true;
+  }
+  // Overwrites n appended bytes at off (before commit).
+  void write(size_t off, const void* src, size_t n) {
+    require(!committed && off + n <= host.size(), FFSIM_ERR_VALUE, "program write out of range");
+    std::memcpy(host.data() + off, src, n);
+  }
+  void commit() {
+    prepare();
+    ProgramRing& R = ctx->ring;
+    const size_t used = host.size();
     if (used) {
       std::memcpy(R.h[slot], host.data(), used);
       HIP_CHECK(hipMemcpyAsync(R.d[slot], R.h[slot], used, hipMemcpyHostToDevice, stream));
@@ -1064,6 +1085,249 @@ size_t append_indices(Program& prog, const uint64_t* idx, int64_t n, int64_t bou
   }
   return prog.append(v.data(), v.size() * sizeof(int64_t));
 }
+
+// The merged rotations of a spin-balanced UCJ operator
+// (python/ffsim/variational/ucj_spin_balanced.py:657-696): V_0 = U_0^H,
+// V_r = U_r^H U_{r-1}, V_{n_reps} = F U_last (or U_last).
+std::vector<std::vector<cdouble>> ucj_merged_rotations(int n, int n_reps, const cdouble* U,
+                                                       const cdouble* final_rot) {
+  const size_t nn = (size_t)n * n;
+  std::vector<std::vector<cdouble>> V(n_reps + 1);
+  std::vector<cdouble> eye(nn, 0.0);
+  for (int k = 0; k < n; ++k) eye[(size_t)k * n + k] = 1.0;
+  for (int r = 0; r < n_reps; ++r)
+    V[r] = matmul(U + r * nn, r == 0 ? eye.data() : U + (r - 1) * nn, n, true);
+  const cdouble* last = n_reps ? U + (n_reps - 1) * nn : eye.data();
+  if (final_rot)
+    V[n_reps] = matmul(final_rot, last, n, false);
+  else
+    V[n_reps] = std::vector<cdouble>(last, last + nn);
+  return V;
+}
+
+void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int nb, int n_reps,
+              const double* diag_coulomb_mats, const double* orbital_rotations,
+              const double* final_orbital_rotation, hipStream_t stream) {
+  const size_t nn = (size_t)n * n;
+  std::vector<std::vector<cdouble>> V = ucj_merged_rotations(
+      n, n_reps, reinterpret_cast<const cdouble*>(orbital_rotations),
+      reinterpret_cast<const cdouble*>(final_orbital_rotation));
+  std::vector<const cdouble*> rots(n_reps + 1);
+  for (int r = 0; r <= n_reps; ++r) rots[r] = V[r].data();
+  std::vector<std::vector<double>> halved(n_reps);
+  std::vector<const double*> mats(3 * n_reps);
+  std::vector<double> times(n_reps, -1.0);
+  for (int r = 0; r < n_reps; ++r) {
+    const double* J = diag_coulomb_mats + r * 2 * nn;
+    halved[r].assign(J, J + nn);
+    for (int k = 0; k < n; ++k) halved[r][(size_t)k * n + k] *= 0.5;
+    mats[3 * r] = halved[r].data();
+    mats[3 * r + 1] = J + nn;
+    mats[3 * r + 2] = halved[r].data();
+  }
+  diag_coulomb_sequence_impl(ctx, vec, work, dtype, n, na, nb, n_reps, rots.data(), mats.data(),
+                             times.data(), 0, stream);
+}
+
+// Largest string count of a sector on the GEMM path: there an orbital rotation
+// costs 8 dim^3 flops per state and sector, against one HBM pass of the state
+// on the Givens path, so only small states take it.
+constexpr long kGemmMaxDim = 256;
+
+// FFSIM_AMD_BATCH_PATH=gemm | sweep forces a path (read at every call, for
+// measurements that compare the two).
+int batch_path_override() {
+  const char* e = std::getenv("FFSIM_AMD_BATCH_PATH");
+  if (!e) return 0;
+  if (std::string(e) == "gemm") return 1;
+  if (std::string(e) == "sweep") return 2;
+  return 0;
+}
+
+bool gemm_path(int norb, int na, int nb) {
+  const long da = (long)binom(norb, na), db = (long)binom(norb, nb);
+  const bool ok = na <= kMaxCompoundK && nb <= kMaxCompoundK && da <= 1024 && db <= 1024;
+  if (batch_path_override() == 1) return ok;
+  if (batch_path_override() == 2) return false;
+  return ok && da <= kGemmMaxDim && db <= kGemmMaxDim;
+}
+
+template <typename T>
+void launch_compound(const Cx<double>* U, int norb, int k, const uint64_t* strings, long dim,
+                     Cx<T>* out, int n_mats, hipStream_t stream) {
+  const dim3 grid((unsigned)((dim * dim + 255) / 256), (unsigned)n_mats);
+  switch (k) {
+#define FFSIM_COMPOUND(KK) \
+  case KK:                \
+    hipLaunchKernelGGL((compound_kernel<T, KK>), grid, dim3(256), 0, stream, U, norb, strings, dim, out); \
+    break;
+    FFSIM_COMPOUND(0)
+    FFSIM_COMPOUND(1)
+    FFSIM_COMPOUND(2)
+    FFSIM_COMPOUND(3)
+    FFSIM_COMPOUND(4)
+    FFSIM_COMPOUND(5)
+    FFSIM_COMPOUND(6)
+#undef FFSIM_COMPOUND
+    default:
+      throw Failure{FFSIM_ERR_VALUE, "compound matrix: too many electrons in a sector"};
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void blas_check(rocblas_status st, const char* what) {
+  require(st == rocblas_status_success, FFSIM_ERR_HIP,
+          std::string(what) + ": rocBLAS status " + rocblas_status_to_string(st));
+}
+
+// One complex GEMM per state, C_i = op(A_i) B_i, column-major, no conjugation.
+template <typename T>
+void gemm_batched(rocblas_handle h, bool trans_a, long m, long n, long k, const void* const* A,
+                  long lda, const void* const* B, long ldb, void* const* C, long ldc, int count) {
+  const rocblas_operation ta = trans_a ? rocblas_operation_transpose : rocblas_operation_none;
+  if constexpr (std::is_same<T, double>::value) {
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    blas_check(rocblas_zgemm_batched(h, ta, rocblas_operation_none, (rocblas_int)m,
+                                     (rocblas_int)n, (rocblas_int)k, &one,
+                                     reinterpret_cast<const rocblas_double_complex* const*>(A),
+                                     (rocblas_int)lda,
+                                     reinterpret_cast<const rocblas_double_complex* const*>(B),
+                                     (rocblas_int)ldb, &zero,
+                                     reinterpret_cast<rocblas_double_complex* const*>(C),
+                                     (rocblas_int)ldc, count),
+               "zgemm_batched");
+  } else {
+    const rocblas_float_complex one{1.0f, 0.0f}, zero{0.0f, 0.0f};
+    blas_check(rocblas_cgemm_batched(h, ta, rocblas_operation_none, (rocblas_int)m,
+                                     (rocblas_int)n, (rocblas_int)k, &one,
+                                     reinterpret_cast<const rocblas_float_complex* const*>(A),
+                                     (rocblas_int)lda,
+                                     reinterpret_cast<const rocblas_float_complex* const*>(B),
+                                     (rocblas_int)ldb, &zero,
+                                     reinterpret_cast<rocblas_float_complex* const*>(C),
+                                     (rocblas_int)ldc, count),
+               "cgemm_batched");
+  }
+}
+
+// A batch of spin-balanced UCJ applications of one shape on the GEMM path:
+// per merged rotation, the compound matrices of every state (one launch per
+// sector), then psi <- L_a psi (into work) and psi <- work L_b^T as batched
+// GEMMs; per diagonal term one batched elementwise pass.
+template <typename T>
+void ucj_batched_gemm(Context& ctx, int n_states, void* const* vecs, void* const* works, int n,
+                      int na, int nb, int n_reps, const double* diag_coulomb_mats,
+                      const double* orbital_rotations, const double* final_rots,
+                      hipStream_t stream) {
+  const long da = (long)binom(n, na), db = (long)binom(n, nb);
+  const size_t nn = (size_t)n * n;
+  const int n_steps = n_reps + 1;
+  const DevStrings& sa = dev_strings(ctx, n, na);
+  const DevStrings& sb = dev_strings(ctx, n, nb);
+  // states per program: compound matrices of every step of a chunk fit the slot's scratch
+  const size_t per_state = (size_t)n_steps * (size_t)(da * da + db * db) * sizeof(Cx<T>);
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_states, (96u << 20) / per_state));
+  for (int i0 = 0; i0 < n_states; i0 += chunk) {
+    const int m = std::min(chunk, n_states - i0);
+    Program prog(ctx, stream);
+    // merged rotations, step-major: U[(step * m + i) * nn]
+    std::vector<cdouble> Us((size_t)n_steps * m * nn);
+    std::vector<double> J((size_t)std::max(n_reps, 1) * m * 2 * nn);
+    for (int i = 0; i < m; ++i) {
+      const int st = i0 + i;
+      std::vector<std::vector<cdouble>> V = ucj_merged_rotations(
+          n, n_reps, reinterpret_cast<const cdouble*>(orbital_rotations) + (size_t)st * n_reps * nn,
+          final_rots ? reinterpret_cast<const cdouble*>(final_rots) + (size_t)st * nn : nullptr);
+      for (int r = 0; r < n_steps; ++r)
+        std::copy(V[r].begin(), V[r].end(), Us.begin() + ((size_t)r * m + i) * nn);
+      for (int r = 0; r < n_reps; ++r) {
+        const double* src = diag_coulomb_mats + ((size_t)st * n_reps + r) * 2 * nn;
+        double* dst = J.data() + ((size_t)r * m + i) * 2 * nn;
+        std::copy(src, src + 2 * nn, dst);
+        for (int k = 0; k < n; ++k) dst[(size_t)k * n + k] *= 0.5;  // same-spin: diagonal halved
+      }
+    }
+    const size_t u_off = prog.append(Us.data(), Us.size() * sizeof(cdouble));
+    const size_t j_off = prog.append(J.data(), J.size() * sizeof(double));
+    size_t la_off = 0, lb_off = 0;
+    require(prog.reserve((size_t)n_steps * m * da * da * sizeof(Cx<T>), &la_off) &&
+                prog.reserve((size_t)n_steps * m * db * db * sizeof(Cx<T>), &lb_off),
+            FFSIM_ERR_VALUE, "batched UCJ: compound matrices exceed the program scratch");
+    // pointer arrays: per step, alpha (A: vec, B: L_a, C: work), beta (A: L_b, B: work,
+    // C: vec), then the states for the diagonal passes; device addresses of the
+    // compound matrices are known after commit, so these are filled below
+    const size_t ptr_count = (size_t)n_steps * 6 * m + m;
+    const size_t p_off = prog.append(nullptr, ptr_count * sizeof(void*));
+    prog.prepare();
+    std::vector<void*> ptrs(ptr_count);
+    Cx<T>* La = prog.scratch<Cx<T>>(la_off);
+    Cx<T>* Lb = prog.scratch<Cx<T>>(lb_off);
+    for (int r = 0; r < n_steps; ++r)
+      for (int i = 0; i < m; ++i) {
+        void** q = ptrs.data() + (size_t)r * 6 * m;
+        q[i] = vecs[i0 + i];
+        q[m + i] = La + ((size_t)r * m + i) * da * da;
+        q[2 * m + i] = works[i0 + i];
+        q[3 * m + i] = Lb + ((size_t)r * m + i) * db * db;
+        q[4 * m + i] = works[i0 + i];
+        q[5 * m + i] = vecs[i0 + i];
+      }
+    for (int i = 0; i < m; ++i) ptrs[(size_t)n_steps * 6 * m + i] = vecs[i0 + i];
+    prog.write(p_off, ptrs.data(), ptr_count * sizeof(void*));
+    prog.commit();
+    const Cx<double>* dU = prog.dev<Cx<double>>(u_off);
+    launch_compound<T>(dU, n, na, sa.d, da, La, n_steps * m, stream);
+    // the beta sector's compound matrices follow those of alpha in the same buffer order
+    launch_compound<T>(dU, n, nb, sb.d, db, Lb, n_steps * m, stream);
+    void** dptr = prog.dev<void*>(p_off);
+    std::lock_guard<std::mutex> lk(ctx.blas_mu);
+    if (!ctx.blas) blas_check(rocblas_create_handle(&ctx.blas), "create_handle");
+    blas_check(rocblas_set_stream(ctx.blas, stream), "set_stream");
+    for (int r = 0; r < n_steps; ++r) {
+      void** q = dptr + (size_t)r * 6 * m;
+      // alpha: X' = X L_a (column-major views of the row-major state and matrix)
+      gemm_batched<T>(ctx.blas, false, db, da, da, (const void* const*)q, db,
+                      (const void* const*)(q + m), da, (void* const*)(q + 2 * m), db, m);
+      // beta: X' = L_b^T X
+      gemm_batched<T>(ctx.blas, true, db, da, db, (const void* const*)(q + 3 * m), db,
+                      (const void* const*)(q + 4 * m), db, (void* const*)(q + 5 * m), db, m);
+      if (r < n_reps) {
+        const dim3 grid((unsigned)((da * db + 255) / 256), (unsigned)m);
+        hipLaunchKernelGGL(diag_batch_kernel<T>, grid, dim3(256), 0, stream,
+                           (Cx<T>* const*)(dptr + (size_t)n_steps * 6 * m),
+                           prog.dev<double>(j_off) + (size_t)r * m * 2 * nn, n, sa.d, da, sb.d,
+                           db, -1.0);
+        HIP_CHECK(hipGetLastError());
+      }
+    }
+    prog.finish();
+  }
+}
+
+void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const* works,
+                      int dtype, int n, int na, int nb, int n_reps,
+                      const double* diag_coulomb_mats, const double* orbital_rotations,
+                      const double* final_rots, hipStream_t stream) {
+  const size_t nn = (size_t)n * n;
+  const long da = (long)binom(n, na), db = (long)binom(n, nb);
+  if (da == 0 || db == 0) return;
+  if (gemm_path(n, na, nb)) {
+    for (int i = 0; i < n_states; ++i)
+      require(works[i] != nullptr, FFSIM_ERR_VALUE, "batched UCJ: a work buffer per state");
+    if (dtype == FFSIM_C128)
+      ucj_batched_gemm<double>(ctx, n_states, vecs, works, n, na, nb, n_reps, diag_coulomb_mats,
+                               orbital_rotations, final_rots, stream);
+    else
+      ucj_batched_gemm<float>(ctx, n_states, vecs, works, n, na, nb, n_reps, diag_coulomb_mats,
+                              orbital_rotations, final_rots, stream);
+    return;
+  }
+  for (int i = 0; i < n_states; ++i)
+    ucj_impl(ctx, vecs[i], works[i], dtype, n, na, nb, n_reps,
+             diag_coulomb_mats + (size_t)i * n_reps * 2 * nn,
+             orbital_rotations + (size_t)i * n_reps * 2 * nn,
+             final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, stream);
+}
 }  // namespace
 }  // namespace ffsim_amd
 
@@ -1348,36 +1612,25 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
     check_sizes(norb, n_alpha, n_beta);
     require(n_reps >= 0, FFSIM_ERR_VALUE, "n_reps must be non-negative");
     Context& ctx = context();
-    const int n = norb;
-    const size_t nn = (size_t)n * n;
-    const cdouble* U = reinterpret_cast<const cdouble*>(orbital_rotations);
-    // V_r = U_r^H U_{r-1} (U_{-1} = I), V_final = F U_last or U_last
-    // (python/ffsim/variational/ucj_spin_balanced.py:657-696)
-    std::vector<std::vector<cdouble>> V(n_reps + 1);
-    std::vector<cdouble> eye(nn, 0.0);
-    for (int k = 0; k < n; ++k) eye[(size_t)k * n + k] = 1.0;
-    for (int r = 0; r < n_reps; ++r)
-      V[r] = matmul(U + r * nn, r == 0 ? eye.data() : U + (r - 1) * nn, n, true);
-    const cdouble* last = n_reps ? U + (n_reps - 1) * nn : eye.data();
-    if (final_orbital_rotation)
-      V[n_reps] = matmul(reinterpret_cast<const cdouble*>(final_orbital_rotation), last, n, false);
-    else
-      V[n_reps] = std::vector<cdouble>(last, last + nn);
-    std::vector<const cdouble*> rots(n_reps + 1);
-    for (int r = 0; r <= n_reps; ++r) rots[r] = V[r].data();
-    std::vector<std::vector<double>> halved(n_reps);
-    std::vector<const double*> mats(3 * n_reps);
-    std::vector<double> times(n_reps, -1.0);
-    for (int r = 0; r < n_reps; ++r) {
-      const double* J = diag_coulomb_mats + r * 2 * nn;
-      halved[r].assign(J, J + nn);
-      for (int k = 0; k < n; ++k) halved[r][(size_t)k * n + k] *= 0.5;
-      mats[3 * r] = halved[r].data();
-      mats[3 * r + 1] = J + nn;
-      mats[3 * r + 2] = halved[r].data();
-    }
-    diag_coulomb_sequence_impl(ctx, vec, work, dtype, n, n_alpha, n_beta, n_reps, rots.data(),
-                               mats.data(), times.data(), 0, (hipStream_t)stream);
+    ucj_impl(ctx, vec, work, dtype, norb, n_alpha, n_beta, n_reps, diag_coulomb_mats,
+             orbital_rotations, final_orbital_rotation, (hipStream_t)stream);
+  });
+}
+
+int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void* const* works,
+                                          int dtype, int norb, int n_alpha, int n_beta,
+                                          int n_reps, const double* diag_coulomb_mats,
+                                          const double* orbital_rotations,
+                                          const double* final_orbital_rotations, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(n_reps >= 0 && n_states >= 0, FFSIM_ERR_VALUE, "negative count");
+    if (n_states == 0) return;
+    Context& ctx = context();
+    ucj_batched_impl(ctx, n_states, vecs, works, dtype, norb, n_alpha, n_beta, n_reps,
+                     diag_coulomb_mats, orbital_rotations, final_orbital_rotations,
+                     (hipStream_t)stream);
   });
 }
 

@@ -1210,3 +1210,126 @@ __global__ void __launch_bounds__(256)
 }
 
 }  // namespace ffsim_amd
+
+namespace ffsim_amd {
+
+// ---------------------------------------------------------------------------
+// Batched small-state path (serving): an orbital rotation as the compound
+// matrix Lambda^k(U) applied by a GEMM, and a batched diagonal pass.
+constexpr int kMaxCompoundK = 6;
+
+// out[m][t][s] = det(U_m[T, S]) with T = strings[t], S = strings[s] (k-subsets of
+// norb orbitals, rows and columns in ascending orbital order): one thread per
+// entry, FP64 Gaussian elimination with partial pivoting.
+template <typename T, int K>
+__global__ void __launch_bounds__(256)
+    compound_kernel(const Cx<double>* __restrict__ U, int norb,
+                    const uint64_t* __restrict__ strings, long dim, Cx<T>* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = blockIdx.y;
+  if (e >= dim * dim) return;
+  Cx<T>* o = out + m * dim * dim + e;
+  if constexpr (K == 0) {
+    *o = Cx<T>{T(1), T(0)};
+  } else {
+    const long t = e / dim, s = e - t * dim;
+    const Cx<double>* u = U + m * norb * norb;
+    int r[K], c[K];
+    uint64_t bt = strings[t], bs = strings[s];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      r[q] = __builtin_ctzll(bt);
+      bt &= bt - 1;
+      c[q] = __builtin_ctzll(bs);
+      bs &= bs - 1;
+    }
+    double ar[K][K], ai[K][K];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const Cx<double> x = u[r[i] * norb + c[j]];
+        ar[i][j] = x.re;
+        ai[i][j] = x.im;
+      }
+    double dr = 1.0, di = 0.0;
+#pragma unroll
+    for (int col = 0; col < K; ++col) {
+      // pivot: the largest |a[row][col]| at or below the diagonal (swapped in place,
+      // the swap flips the sign of the determinant)
+#pragma unroll
+      for (int row = col + 1; row < K; ++row) {
+        const double mp = ar[col][col] * ar[col][col] + ai[col][col] * ai[col][col];
+        const double mr = ar[row][col] * ar[row][col] + ai[row][col] * ai[row][col];
+        if (mr > mp) {
+#pragma unroll
+          for (int j = 0; j < K; ++j) {
+            const double tr = ar[col][j], ti = ai[col][j];
+            ar[col][j] = ar[row][j];
+            ai[col][j] = ai[row][j];
+            ar[row][j] = tr;
+            ai[row][j] = ti;
+          }
+          dr = -dr;
+          di = -di;
+        }
+      }
+      const double pr = ar[col][col], pi = ai[col][col];
+      const double t0 = dr * pr - di * pi;
+      di = dr * pi + di * pr;
+      dr = t0;
+      const double p2 = pr * pr + pi * pi;
+      if (p2 == 0.0) {
+        dr = di = 0.0;
+        break;
+      }
+      const double ir = pr / p2, ii = -pi / p2;  // 1 / pivot
+#pragma unroll
+      for (int row = col + 1; row < K; ++row) {
+        const double fr = ar[row][col] * ir - ai[row][col] * ii;
+        const double fi = ar[row][col] * ii + ai[row][col] * ir;
+#pragma unroll
+        for (int j = col + 1; j < K; ++j) {
+          ar[row][j] -= fr * ar[col][j] - fi * ai[col][j];
+          ai[row][j] -= fr * ai[col][j] + fi * ar[col][j];
+        }
+      }
+    }
+    *o = Cx<T>{(T)dr, (T)di};
+  }
+}
+
+// psi_i[a][b] *= exp(-i t E_i(a, b)) for a batch of states of one shape, number
+// representation, one same-spin matrix A_i (diagonal halved) for both sectors
+// and the opposite-spin matrix X_i: E = sum_{j <= k in a} A[j][k] +
+// sum_{j <= k in b} A[j][k] + sum_{p in a, q in b} X[p][q].  mats holds A_i then
+// X_i per state; one thread per element, state i = blockIdx.y.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    diag_batch_kernel(Cx<T>* const* __restrict__ states, const double* __restrict__ mats,
+                      int norb, const uint64_t* __restrict__ str_a, long dim_a,
+                      const uint64_t* __restrict__ str_b, long dim_b, double time) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= dim_a * dim_b) return;
+  const long a = e / dim_b, b = e - a * dim_b;
+  const double* A = mats + (long)blockIdx.y * 2 * norb * norb;
+  const double* X = A + norb * norb;
+  const uint64_t sa = str_a[a], sb = str_b[b];
+  double E = 0.0;
+  for (uint64_t x = sa; x; x &= x - 1) {
+    const int j = __builtin_ctzll(x);
+    for (uint64_t y = x; y; y &= y - 1) E += A[j * norb + __builtin_ctzll(y)];
+    for (uint64_t y = sb; y; y &= y - 1) E += X[j * norb + __builtin_ctzll(y)];
+  }
+  for (uint64_t x = sb; x; x &= x - 1) {
+    const int j = __builtin_ctzll(x);
+    for (uint64_t y = x; y; y &= y - 1) E += A[j * norb + __builtin_ctzll(y)];
+  }
+  double sn, cs;
+  sincos(-time * E, &sn, &cs);
+  Cx<T>* p = states[blockIdx.y] + e;
+  const Cx<T> v = *p;
+  *p = Cx<T>{(T)(v.re * cs - v.im * sn), (T)(v.re * sn + v.im * cs)};
+}
+
+}  // namespace ffsim_amd

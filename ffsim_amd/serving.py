@@ -19,6 +19,12 @@ requests in arrival order when both a stream slot and enough HBM are free:
   same as when it runs alone, because every kernel is deterministic and a
   request's work stays on one stream.
 
+* requests admitted in the same scheduling round that share a shape
+  (norb, nelec, dtype, n_reps, final rotation or not) are launched together as
+  one batch (``ffsim_apply_ucj_spin_balanced_batched``): small states take the
+  compound-matrix GEMM path, one set of launches for the whole batch; larger
+  ones run back to back on the batch's stream.
+
 Latency is measured from arrival (submission) to completion seen by the
 scheduler (event query); :meth:`LUCJServer.stats` reports p50/p99 and
 throughput.  The device work itself is ``UCJOpSpinBalanced._apply_unitary_``
@@ -96,14 +102,18 @@ class Ticket:
 
 
 def request_footprint(norb: int, nelec, dtype: str) -> int:
-    """HBM bytes a request holds while in flight: state + work buffer."""
-    from ffsim_amd import _lib
-
+    """HBM bytes a request holds while in flight: its state and a work buffer of
+    the state's size (the batched path's GEMMs run out of place), each rounded up
+    to the allocator granule."""
     elem = 16 if dtype == "c128" else 8
-    tag = _lib.C128 if dtype == "c128" else _lib.C64
     n = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
-    work = _lib.check(_lib.lib().ffsim_orbital_rotation_work_elements(tag, norb, *nelec))
-    return sum(-(-b // _GRANULE) * _GRANULE for b in (n * elem, work * elem) if b)
+    return 2 * (-(-(n * elem) // _GRANULE) * _GRANULE)
+
+
+def batch_key(req: "LUCJRequest"):
+    """Requests with equal keys can share one batched launch."""
+    return (req.norb, tuple(req.nelec), req.dtype, req.op.n_reps,
+            req.op.final_orbital_rotation is not None)
 
 
 class DeviceBackend:
@@ -126,6 +136,49 @@ class DeviceBackend:
 
     def footprint(self, req: LUCJRequest) -> int:
         return request_footprint(req.norb, tuple(req.nelec), req.dtype)
+
+    def _device_state(self, req: LUCJRequest):
+        torch = self.torch
+        from ffsim_amd.states import hartree_fock_state
+
+        cdt = torch.complex128 if req.dtype == "c128" else torch.complex64
+        if isinstance(req.state, torch.Tensor) and req.state.device.type == "cuda":
+            return req.state.reshape(-1).to(cdt, copy=True)
+        host = req.state if req.state is not None else hartree_fock_state(req.norb, req.nelec)
+        host = torch.from_numpy(np.ascontiguousarray(host, dtype=np.complex128)).to(cdt)
+        return self._device.host_to_device(host.numpy(), self.device)
+
+    batch_key = staticmethod(batch_key)
+
+    def launch_batch(self, reqs: list, stream_index: int):
+        """One batched application for requests of one shape (see batch_key)."""
+        torch = self.torch
+        from ffsim_amd.variational.ucj_spin_balanced import UCJOpSpinBalanced
+
+        r0 = reqs[0]
+        stream = self.streams[stream_index]
+        with torch.cuda.stream(stream):
+            states = [self._device_state(r) for r in reqs]
+            works = [torch.empty_like(t) for t in states]
+            UCJOpSpinBalanced.apply_batched_in_place([r.op for r in reqs], states, r0.norb,
+                                                     tuple(r0.nelec), works)
+            staged = [self._device.start_device_to_host(t) if r.to_host else None
+                      for r, t in zip(reqs, states)]
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return (states, works, ev, stream, staged)
+
+    def collect_batch(self, reqs: list, handle) -> list:
+        states, works, ev, stream, staged = handle
+        ev.synchronize()
+        out = []
+        for t, st in zip(states, staged):
+            if st is not None:
+                out.append(self._device.finish_device_to_host(st))
+            else:
+                t.record_stream(self.torch.cuda.default_stream(self.device))
+                out.append(t)
+        return out
 
     def launch(self, req: LUCJRequest, stream_index: int):
         torch = self.torch
@@ -169,19 +222,28 @@ class DeviceBackend:
 
 
 class LUCJServer:
-    """Scheduler: admission by HBM footprint, continuous refill of in-flight slots."""
+    """Scheduler: admission by HBM footprint, continuous refill of in-flight slots.
+
+    Each slot holds one launch: a single request, or, with a backend that
+    batches (``launch_batch``), up to ``max_batch`` requests of one shape
+    admitted in the same scheduling round."""
 
     def __init__(self, *, hbm_budget: int, slots_per_stream: int = 2, backend=None,
-                 n_streams: int = 4, max_bypass: int = 8, poll_interval: float = 2e-5):
+                 n_streams: int = 4, max_bypass: int = 8, poll_interval: float = 2e-5,
+                 max_batch: int = 32):
         self.backend = backend if backend is not None else DeviceBackend(n_streams)
         self.hbm_budget = int(hbm_budget)
-        self.max_inflight = slots_per_stream * self.backend.n_streams
+        self.batching = hasattr(self.backend, "launch_batch")
+        self.max_batch = max(1, int(max_batch)) if self.batching else 1
+        self.max_launches = slots_per_stream * self.backend.n_streams
+        self.max_inflight = self.max_launches * self.max_batch
         self.max_bypass = max_bypass
         self.poll_interval = poll_interval
         self._lock = threading.Lock()
         self._ids = itertools.count()
         self._queue: collections.deque[Ticket] = collections.deque()
-        self._inflight: list[tuple[Ticket, Any, int]] = []
+        # (tickets of one launch, backend handle, stream index)
+        self._inflight: list[tuple[list[Ticket], Any, int]] = []
         self._stream_load = [0] * self.backend.n_streams
         self.bytes_in_use = 0
         self.peak_bytes = 0
@@ -212,10 +274,13 @@ class LUCJServer:
         return True
 
     # -- scheduler ---------------------------------------------------------
+    def _in_flight_requests(self) -> int:
+        return sum(len(g) for g, _, _ in self._inflight)
+
     def _admit(self) -> list[Ticket]:
         admitted = []
         with self._lock:
-            free_slots = self.max_inflight - len(self._inflight)
+            free_slots = self.max_inflight - self._in_flight_requests()
             kept = collections.deque()
             waiting: list[Ticket] = []  # earlier requests that do not fit yet
             while self._queue and free_slots > 0:
@@ -237,51 +302,80 @@ class LUCJServer:
             self._queue = kept
         return admitted
 
+    def _groups(self, tickets: list[Ticket]) -> list[list[Ticket]]:
+        """Launch groups: one per request, or same-shape batches of at most max_batch."""
+        if not self.batching:
+            return [[t] for t in tickets]
+        by_key: dict = {}
+        for t in tickets:
+            by_key.setdefault(self.backend.batch_key(t.request), []).append(t)
+        groups = []
+        for members in by_key.values():
+            for k in range(0, len(members), self.max_batch):
+                groups.append(members[k:k + self.max_batch])
+        return groups
+
+    def _launch(self, group: list[Ticket]) -> None:
+        s = min(range(self.backend.n_streams), key=lambda k: self._stream_load[k])
+        now = time.perf_counter()
+        for t in group:
+            t.t_admit = now
+        try:
+            if self.batching:
+                handle = self.backend.launch_batch([t.request for t in group], s)
+            else:
+                handle = self.backend.launch(group[0].request, s)
+        except BaseException as e:  # noqa: BLE001 - reported on the tickets
+            with self._lock:
+                self.bytes_in_use -= sum(t.footprint for t in group)
+            for t in group:
+                t._finish(error=e)
+                self.completed.append(t)
+            return
+        self._stream_load[s] += 1
+        self._inflight.append((group, handle, s))
+
     def step(self) -> int:
         """One scheduling round: retire finished work, admit and launch. Returns the
         number of requests still queued or in flight."""
         still = []
-        for t, handle, s in self._inflight:
+        for group, handle, s in self._inflight:
             try:
                 finished = self.backend.done(handle)
-            except BaseException as e:  # noqa: BLE001 - reported on the ticket
-                self._retire(t, handle, s, error=e)
+            except BaseException as e:  # noqa: BLE001 - reported on the tickets
+                self._retire(group, handle, s, error=e)
                 continue
             if finished:
                 try:
-                    value = self.backend.collect(t.request, handle)
+                    if self.batching:
+                        values = self.backend.collect_batch([t.request for t in group], handle)
+                    else:
+                        values = [self.backend.collect(group[0].request, handle)]
                 except BaseException as e:  # noqa: BLE001
-                    self._retire(t, handle, s, error=e)
+                    self._retire(group, handle, s, error=e)
                 else:
-                    self._retire(t, handle, s, value=value)
+                    self._retire(group, handle, s, values=values)
             else:
-                still.append((t, handle, s))
+                still.append((group, handle, s))
         self._inflight = still
-        for t in self._admit():
-            s = min(range(self.backend.n_streams), key=lambda k: self._stream_load[k])
-            t.t_admit = time.perf_counter()
-            try:
-                handle = self.backend.launch(t.request, s)
-            except BaseException as e:  # noqa: BLE001
-                with self._lock:
-                    self.bytes_in_use -= t.footprint
-                t._finish(error=e)
-                self.completed.append(t)
-                continue
-            self._stream_load[s] += 1
-            self._inflight.append((t, handle, s))
+        admitted = self._admit()
+        # a launch slot per group: batches of one shape, at most max_launches in flight
+        for group in self._groups(admitted):
+            self._launch(group)
         with self._lock:
-            return len(self._queue) + len(self._inflight)
+            return len(self._queue) + self._in_flight_requests()
 
-    def _retire(self, t: Ticket, handle, s: int, value=None, error=None):
+    def _retire(self, group: list[Ticket], handle, s: int, values=None, error=None):
         try:
             self.backend.release(handle)
         finally:
             self._stream_load[s] -= 1
             with self._lock:
-                self.bytes_in_use -= t.footprint
-            t._finish(value=value, error=error)
-            self.completed.append(t)
+                self.bytes_in_use -= sum(t.footprint for t in group)
+            now = time.perf_counter()
+            for k, t in enumerate(group):
+                t._finish(value=None if values is None else values[k], error=error, now=now)
+                self.completed.append(t)
 
     def drain(self, timeout: float = 600.0) -> None:
         """Run the scheduler until nothing is queued or in flight."""

@@ -8,6 +8,8 @@ rotation, one elementwise pass per diagonal Coulomb evolution).
 
 from __future__ import annotations
 
+import ctypes
+
 import itertools
 from dataclasses import InitVar, dataclass
 
@@ -262,6 +264,45 @@ class UCJOpSpinBalanced:
             _device.stream_ptr(),
         )
         return t
+
+    @staticmethod
+    def apply_batched_in_place(ops, states, norb: int, nelec, works=None) -> list:
+        """Apply ``ops[i]`` to the resident 1-D device state ``states[i]`` in place, for
+        states of one shape and dtype, in one C-ABI call (batched launches for small
+        states).  ``works``: optional per-state device buffers of the state's size
+        and dtype; allocated here when not given.  Operators must share ``n_reps``
+        and whether they have a final orbital rotation."""
+        ops, states = list(ops), list(states)
+        if len(ops) != len(states):
+            raise ValueError("one operator per state")
+        if not states:
+            return states
+        na, nb = nelec
+        n_reps = ops[0].n_reps
+        has_final = ops[0].final_orbital_rotation is not None
+        tag = _device.dtype_tag(states[0])
+        for op, t in zip(ops, states):
+            _device.check_size(t, norb, nelec)
+            if op.n_reps != n_reps or (op.final_orbital_rotation is not None) != has_final:
+                raise ValueError("a batch needs operators with the same n_reps and final-rotation form")
+            if op.norb != norb:
+                raise ValueError(f"operator has norb={op.norb}, expected {norb}")
+            if _device.dtype_tag(t) != tag or t.device != states[0].device:
+                raise ValueError("a batch needs states of one dtype on one device")
+        if works is None:
+            works = [torch.empty_like(t) for t in states]
+        dcm = np.ascontiguousarray(np.stack([op.diag_coulomb_mats for op in ops]), dtype=float)
+        rots = np.ascontiguousarray(np.stack([op.orbital_rotations for op in ops]),
+                                    dtype=np.complex128)
+        final = (np.ascontiguousarray(np.stack([op.final_orbital_rotation for op in ops]),
+                                      dtype=np.complex128) if has_final else None)
+        vec_ptrs = (ctypes.c_void_p * len(states))(*[t.data_ptr() for t in states])
+        work_ptrs = (ctypes.c_void_p * len(states))(
+            *[w.data_ptr() if w is not None else None for w in works])
+        _lib.call("ffsim_apply_ucj_spin_balanced_batched", len(states), vec_ptrs, work_ptrs,
+                  tag, norb, na, nb, n_reps, _device.host_ptr(dcm), _device.host_ptr(rots),
+                  _device.host_ptr(final), _device.stream_ptr())
+        return states
 
     def _apply_unitary_(self, vec, norb: int, nelec, copy: bool):
         if isinstance(nelec, int):

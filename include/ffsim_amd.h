@@ -138,6 +138,27 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
                                   const double* orbital_rotations,
                                   const double* final_orbital_rotation, void* stream);
 
+/* UCJOpSpinBalanced._apply_unitary_ (python/ffsim/variational/ucj_spin_balanced.py:
+ * 657-696) for a batch of n_states states of one shape (norb, n_alpha, n_beta,
+ * dtype), each with its own operator; batching has no counterpart in the
+ * reference (its serving unit is one call per state).  vecs / works: HOST
+ * arrays of n_states DEVICE pointers (states and their work buffers, each
+ * dim_a * dim_b elements; works may hold NULLs where
+ * ffsim_orbital_rotation_work_elements is 0 and the GEMM path below is not
+ * taken).  diag_coulomb_mats: n_states x n_reps x 2 x norb x norb;
+ * orbital_rotations: n_states x n_reps x norb x norb complex128;
+ * final_orbital_rotations: NULL or n_states x norb x norb.  Small states
+ * (at most 6 electrons and 256 strings per sector) take a GEMM path: each
+ * merged orbital rotation is the compound matrix Lambda^k(U) (built on the
+ * device) applied to all states by batched complex GEMMs, and each diagonal
+ * term one batched elementwise pass; larger states run the Givens path state
+ * by state on the stream. */
+int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void* const* works,
+                                          int dtype, int norb, int n_alpha, int n_beta,
+                                          int n_reps, const double* diag_coulomb_mats,
+                                          const double* orbital_rotations,
+                                          const double* final_orbital_rotations, void* stream);
+
 /* One-body contraction: out (+)= scale * sum_{pq, sigma} mat[p][q] a+_{p sigma} a_{q sigma} vec,
  * mat a HOST norb x norb complex128 matrix.  Replaces contract_one_body
  * (python/ffsim/contract/one_body.py:24-56, i.e. pyscf's contract_1e over the link

@@ -1,5 +1,7 @@
 """Serving on the GPU: mixed shapes in flight together give the same results
-as each request alone, and the HBM budget holds.
+as each request alone (within the stated tolerance: same-shape requests are
+batched, and small states then take the compound-matrix GEMM path, whose
+rounding differs from the Givens path's), and the HBM budget holds.
 
 Every test prints the case it runs (flushed) before touching the GPU.
 """
@@ -27,18 +29,15 @@ def _lucj_pairs(norb):
 
 
 def test_request_footprint_counts_state_and_work_buffer():
-    from ffsim_amd import _lib
     from ffsim_amd.serving import request_footprint
 
     _case("footprint")
     for norb, nelec, dtype in [(8, (4, 4), "c128"), (12, (6, 5), "c64"), (14, (7, 7), "c128")]:
         elem = 16 if dtype == "c128" else 8
         n = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
-        tag = _lib.C128 if dtype == "c128" else _lib.C64
-        work = _lib.check(_lib.lib().ffsim_orbital_rotation_work_elements(tag, norb, *nelec))
         fp = request_footprint(norb, nelec, dtype)
-        assert fp >= (n + work) * elem
-        assert fp < (n + work) * elem + 2 * (2 << 20) + 1
+        assert fp >= 2 * n * elem  # the state and a work buffer of its size
+        assert fp < 2 * n * elem + 2 * (2 << 20) + 1
 
 
 def _mixed_requests(rng, n, shapes):
@@ -76,13 +75,14 @@ def test_results_identical_alone_and_in_mix(rng):
         r.op.apply_in_place(t, r.norb, r.nelec)
         alone.append(t.cpu().numpy())
     torch.cuda.synchronize()
-    server = LUCJServer(hbm_budget=64 << 20, slots_per_stream=2, n_streams=4)
+    server = LUCJServer(hbm_budget=64 << 20, slots_per_stream=2, n_streams=4, max_batch=8)
     tickets = [server.submit(r) for r in reqs]
     server.drain(timeout=120)
     for r, t, a in zip(reqs, tickets, alone):
         got = t.result()
         assert got.dtype == a.dtype
-        np.testing.assert_array_equal(got, a)  # bitwise: deterministic kernels, one stream each
+        tol = 1e-12 if r.dtype == "c128" else 2e-6
+        np.testing.assert_allclose(got, a, atol=tol, rtol=0)
     # and the c128 results match the oracle
     for r, t in zip(reqs[:6], tickets[:6]):
         if r.dtype != "c128":
@@ -95,6 +95,37 @@ def test_results_identical_alone_and_in_mix(rng):
     print("serving stats", st, flush=True)
     assert st["served"] == len(reqs) and st["failed"] == 0
     assert st["peak_hbm_bytes"] <= 64 << 20
+    # the same requests in another mix (other batch compositions) agree as well
+    server2 = LUCJServer(hbm_budget=64 << 20, slots_per_stream=1, n_streams=2, max_batch=3)
+    tickets2 = [server2.submit(r) for r in reversed(reqs)]
+    server2.drain(timeout=120)
+    for r, t, t2 in zip(reqs, tickets, reversed(tickets2)):
+        tol = 1e-12 if r.dtype == "c128" else 2e-6
+        np.testing.assert_allclose(t2.result(), t.result(), atol=tol, rtol=0)
+
+
+def test_device_resident_requests_stay_on_the_device(rng):
+    import torch
+
+    from ffsim_amd.serving import LUCJRequest, LUCJServer
+
+    _case("serving device-resident")
+    reqs = _mixed_requests(rng, 6, SHAPES[:3])
+    dev_reqs = []
+    for r in reqs:
+        cdt = torch.complex128 if r.dtype == "c128" else torch.complex64
+        dev_reqs.append(LUCJRequest(op=r.op, norb=r.norb, nelec=r.nelec, dtype=r.dtype,
+                                    state=torch.from_numpy(r.state).to(cdt).cuda(), to_host=False))
+    server = LUCJServer(hbm_budget=64 << 20, slots_per_stream=2, n_streams=2)
+    tickets = [server.submit(r) for r in dev_reqs]
+    server.drain(timeout=120)
+    for r, t in zip(reqs, tickets):
+        got = t.result()
+        assert isinstance(got, torch.Tensor) and got.device.type == "cuda"
+        expected = O.apply_ucj_spin_balanced(r.state, r.op.diag_coulomb_mats,
+                                             r.op.orbital_rotations, r.op.final_orbital_rotation,
+                                             r.norb, r.nelec)
+        np.testing.assert_allclose(got.cpu().numpy(), expected, atol=1e-12, rtol=0)
 
 
 def test_tight_budget_serializes_large_requests(rng):

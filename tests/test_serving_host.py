@@ -169,3 +169,58 @@ def test_background_thread_serves_concurrent_submissions():
     finally:
         server.stop()
     assert server.stats()["served"] == 50
+
+
+class FakeBatchBackend(FakeBackend):
+    """Batches requests of one key (the job's `key` attribute) into one launch."""
+
+    def __init__(self, n_streams=2):
+        super().__init__(n_streams)
+        self.batches = []
+
+    @staticmethod
+    def batch_key(req):
+        return req.op.key
+
+    def launch_batch(self, reqs, stream_index):
+        assert len({r.op.key for r in reqs}) == 1, "a batch mixes shapes"
+        handles = [self.launch(r, stream_index) for r in reqs]
+        self.batches.append([r.op.key for r in reqs])
+        return {"handles": handles, "left": max(h["left"] for h in handles), "stream": stream_index,
+                "job": reqs[0].op}
+
+    def done(self, handle):
+        if "handles" not in handle:
+            return super().done(handle)
+        for h in handle["handles"]:
+            if h["job"].fail_run:
+                raise RuntimeError("injected device failure")
+        handle["left"] -= 1
+        return handle["left"] <= 0
+
+    def collect_batch(self, reqs, handle):
+        return [("result", id(r)) for r in reqs]
+
+    def release(self, handle):
+        for h in handle["handles"]:
+            super().release(h)
+
+
+def test_batches_group_one_shape_and_respect_the_budget():
+    rng = np.random.default_rng(9)
+    backend = FakeBatchBackend(n_streams=2)
+    server = LUCJServer(hbm_budget=400, slots_per_stream=2, backend=backend, poll_interval=0.0,
+                        max_batch=5)
+    jobs = []
+    for _ in range(80):
+        job = FakeJob(int(rng.integers(5, 40)), int(rng.integers(1, 4)))
+        job.key = int(rng.integers(0, 3))
+        jobs.append(job)
+    tickets = [server.submit(_req(j)) for j in jobs]
+    while server.step():
+        assert server.bytes_in_use <= 400
+        assert backend.live_bytes <= 400
+    assert all(t.result() == ("result", id(t.request)) for t in tickets)
+    assert all(len(b) <= 5 and len(set(b)) == 1 for b in backend.batches)
+    assert max(len(b) for b in backend.batches) > 1  # requests were batched
+    assert len(server.completed) == 80

@@ -116,6 +116,18 @@ def batch_key(req: "LUCJRequest"):
             req.op.final_orbital_rotation is not None)
 
 
+@dataclass
+class _Launch:
+    """Device work of one launch: its states (results), their work buffers, the
+    event recorded behind it, its stream and the pinned copies of host results."""
+
+    states: list
+    works: list
+    event: Any
+    stream: Any
+    staged: list
+
+
 class DeviceBackend:
     """Runs requests on the GPU: one torch stream per slot group."""
 
@@ -166,13 +178,12 @@ class DeviceBackend:
                       for r, t in zip(reqs, states)]
             ev = torch.cuda.Event()
             ev.record(stream)
-        return (states, works, ev, stream, staged)
+        return _Launch(states, works, ev, stream, staged)
 
-    def collect_batch(self, reqs: list, handle) -> list:
-        states, works, ev, stream, staged = handle
-        ev.synchronize()
+    def collect_batch(self, reqs: list, handle: _Launch) -> list:
+        handle.event.synchronize()
         out = []
-        for t, st in zip(states, staged):
+        for t, st in zip(handle.states, handle.staged):
             if st is not None:
                 out.append(self._device.finish_device_to_host(st))
             else:
@@ -181,27 +192,24 @@ class DeviceBackend:
         return out
 
     def launch(self, req: LUCJRequest, stream_index: int):
+        """One request alone (the unbatched form of launch_batch)."""
         torch = self.torch
-        from ffsim_amd.states import hartree_fock_state
 
-        cdt = torch.complex128 if req.dtype == "c128" else torch.complex64
-        host = req.state if req.state is not None else hartree_fock_state(req.norb, req.nelec)
-        host = torch.from_numpy(np.ascontiguousarray(host, dtype=np.complex128)).to(cdt)
         stream = self.streams[stream_index]
         with torch.cuda.stream(stream):
-            t = self._device.host_to_device(host.numpy(), self.device)
+            t = self._device_state(req)
             req.op.apply_in_place(t, req.norb, tuple(req.nelec))
             # the result's copy to pinned host memory is queued behind the kernels,
             # so a finished request needs only a host-side copy to be collected
             staged = self._device.start_device_to_host(t) if req.to_host else None
             ev = torch.cuda.Event()
             ev.record(stream)
-        return (t, ev, stream, staged)
+        return _Launch([t], [], ev, stream, [staged])
 
-    def done(self, handle) -> bool:
-        return handle[1].query()
+    def done(self, handle: _Launch) -> bool:
+        return handle.event.query()
 
-    def collect(self, req: LUCJRequest, handle):
+    def collect(self, req: LUCJRequest, handle: _Launch):
         """The result: a numpy vector, or (``to_host=False``) the device tensor.
 
         A device result was allocated on the server's side stream and is complete
@@ -210,12 +218,7 @@ class DeviceBackend:
         work queued there still reads it; a caller that reads it on yet another
         stream calls ``tensor.record_stream(that_stream)`` first.
         """
-        t, ev, stream, staged = handle
-        ev.synchronize()
-        if staged is not None:
-            return self._device.finish_device_to_host(staged)
-        t.record_stream(self.torch.cuda.default_stream(self.device))
-        return t
+        return self.collect_batch([req], handle)[0]
 
     def release(self, handle) -> None:
         pass

@@ -1129,10 +1129,17 @@ void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int
                              times.data(), 0, stream);
 }
 
-// Largest string count of a sector on the GEMM path: there an orbital rotation
-// costs 8 dim^3 flops per state and sector, against one HBM pass of the state
-// on the Givens path, so only small states take it.
-constexpr long kGemmMaxDim = 256;
+// Where the GEMM path is taken by default.  An orbital rotation there costs
+// 8 dim^3 flops per state and sector against one HBM pass of the state on the
+// Givens path, and the path's fixed cost (a compound-matrix launch and two
+// batched GEMMs per rotation) is shared by the batch.  Measured per state
+// (LUCJ n_reps=2, complex128, MI355X): norb=8 (70 strings) 0.15 vs 0.24 ms
+// alone and 0.014 vs 0.21 ms in a batch of 32; norb=10 (252 strings) 0.62 vs
+// 0.31 ms alone but 0.078 vs 0.25 ms in a batch of 32; norb=12 (924 strings)
+// 2.6 vs 0.44 ms even in a batch of 32.
+constexpr long kGemmAlwaysDim = 100;   // any batch
+constexpr long kGemmBatchDim = 256;    // batches of at least kGemmMinBatch states
+constexpr int kGemmMinBatch = 8;
 
 // FFSIM_AMD_BATCH_PATH=gemm | sweep forces a path (read at every call, for
 // measurements that compare the two).
@@ -1144,12 +1151,13 @@ int batch_path_override() {
   return 0;
 }
 
-bool gemm_path(int norb, int na, int nb) {
+bool gemm_path(int norb, int na, int nb, int n_states) {
   const long da = (long)binom(norb, na), db = (long)binom(norb, nb);
   const bool ok = na <= kMaxCompoundK && nb <= kMaxCompoundK && da <= 1024 && db <= 1024;
   if (batch_path_override() == 1) return ok;
   if (batch_path_override() == 2) return false;
-  return ok && da <= kGemmMaxDim && db <= kGemmMaxDim;
+  const long d = std::max(da, db);
+  return ok && (d <= kGemmAlwaysDim || (d <= kGemmBatchDim && n_states >= kGemmMinBatch));
 }
 
 template <typename T>
@@ -1311,7 +1319,7 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
   const size_t nn = (size_t)n * n;
   const long da = (long)binom(n, na), db = (long)binom(n, nb);
   if (da == 0 || db == 0) return;
-  if (gemm_path(n, na, nb)) {
+  if (gemm_path(n, na, nb, n_states)) {
     for (int i = 0; i < n_states; ++i)
       require(works[i] != nullptr, FFSIM_ERR_VALUE, "batched UCJ: a work buffer per state");
     if (dtype == FFSIM_C128)

@@ -31,13 +31,16 @@ def _pairs(norb):
     return [(p, p + 1) for p in range(norb - 1)], [(p, p) for p in range(norb)]
 
 
+@pytest.mark.parametrize("path", ["auto", "gemm", "sweep"])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize(
     "norb,nelec,n_states,final",
     [(4, (2, 1), 3, False), (6, (3, 3), 5, True), (8, (4, 4), 7, False), (8, (4, 3), 4, True),
      (10, (5, 5), 6, False), (10, (4, 5), 3, True), (12, (6, 6), 3, False), (13, (6, 5), 2, True)],
 )
-def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype):
+def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype, path, monkeypatch):
+    """path: the library's choice, or FFSIM_AMD_BATCH_PATH forcing the GEMM path
+    (taken up to 1024 strings per sector) or the Givens path."""
     import torch
 
     import ffsim_amd
@@ -50,7 +53,9 @@ def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype):
     dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
     vecs = [O.random_state_vector(dim, rng) for _ in range(n_states)]
     cdt = torch.complex128 if dtype == "c128" else torch.complex64
-    _case("batched ucj", norb, nelec, n_states, final, dtype)
+    if path != "auto":
+        monkeypatch.setenv("FFSIM_AMD_BATCH_PATH", path)
+    _case("batched ucj", norb, nelec, n_states, final, dtype, path)
     states = [torch.from_numpy(v).to("cuda").to(cdt) for v in vecs]
     single = [s.clone() for s in states]
     UCJOpSpinBalanced.apply_batched_in_place(ops, states, norb, nelec)

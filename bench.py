@@ -320,10 +320,67 @@ def gates_leg(norb: int = 16, reps: int = 5, seed: int = 21) -> dict:
                 lambda: ffsim_amd.apply_num_op_sum_evolution(psi, lam, 1.0, norb, nelec,
                                                              copy=False), reps),
         }
+        out[tag]["native_calls"] = _native_calls(psi.view(da, db), norb, nelec[0], rng, reps)
         log(f"gates {tag}: {out[tag]}")
         del psi
         torch.cuda.empty_cache()
     return out
+
+
+def _native_calls(vec, norb: int, nocc: int, rng, reps: int) -> dict:
+    """ffsim's per-rotation calls on the resident da x db state (alpha strings on
+    rows): one Givens rotation on orbitals (norb/2 - 1, norb/2) with ffsim's
+    pair tables, one phase shift of the rows with orbital 0 occupied, and a
+    layer of norb/2 disjoint rotations (0,1), (2,3), ... as norb/2 single calls
+    and as one ffsim_apply_givens_layer call.  GB/s = bytes of the touched rows
+    read and written over the wall time per call."""
+    import torch
+
+    from ffsim_amd import _cistring, native
+
+    row_bytes = vec.shape[1] * vec.element_size()
+    mid = norb // 2
+    idx = _cistring.zero_one_subspace_indices(norb, nocc, (mid - 1, mid))
+    half = len(idx) // 2
+    s1, s2 = idx[:half], idx[half:]
+    strings = _cistring.make_strings(range(norb), nocc)
+    occ0 = np.flatnonzero(strings & 1).astype(np.uint64)
+    theta = rng.uniform(0, 2 * np.pi, size=mid)
+    phis = rng.uniform(0, 2 * np.pi, size=mid)
+    layer = [(math.cos(t), math.sin(t) * complex(math.cos(p), math.sin(p)), 2 * k, 2 * k + 1)
+             for k, (t, p) in enumerate(zip(theta, phis))]
+    layer_tables = [_cistring.zero_one_subspace_indices(norb, nocc, (i, j))
+                    for _, _, i, j in layer]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def singles():
+        for (c, s, _, _), t in zip(layer, layer_tables):
+            h = len(t) // 2
+            native.apply_givens_rotation_in_place(vec, c, s, t[:h], t[h:])
+
+    c, s = layer[0][0], layer[0][1]
+    t_rot = timed(lambda: native.apply_givens_rotation_in_place(vec, c, s, s1, s2))
+    t_phase = timed(lambda: native.apply_phase_shift_in_place(vec, complex(0.6, 0.8), occ0))
+    t_singles = timed(singles)
+    t_layer = timed(lambda: native.apply_givens_layer(vec, layer, norb, nocc))
+    layer_rows = sum(len(t) for t in layer_tables)
+    return {
+        "givens_rotation": {"ms_per_call": round(t_rot * 1e3, 3), "rows": 2 * half,
+                            "GBs": round(2 * 2 * half * row_bytes / t_rot / 1e9, 1)},
+        "phase_shift": {"ms_per_call": round(t_phase * 1e3, 3), "rows": len(occ0),
+                        "GBs": round(2 * len(occ0) * row_bytes / t_phase / 1e9, 1)},
+        "layer_as_single_calls": {"ms": round(t_singles * 1e3, 3), "calls": len(layer),
+                                  "GBs": round(2 * layer_rows * row_bytes / t_singles / 1e9, 1)},
+        "layer_one_call": {"ms": round(t_layer * 1e3, 3), "rotations": len(layer)},
+    }
 
 
 def numpy_io_leg(op, norb: int, nelec, reps: int = 3, seed: int = 0) -> dict:

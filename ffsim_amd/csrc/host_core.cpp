@@ -603,6 +603,33 @@ GivensDecomposition givens_decomposition(const cdouble* mat, int n, double tol) 
   return out;
 }
 
+GivensDecomposition triangle_decomposition(const cdouble* mat, int n, double tol) {
+  GivensDecomposition out;
+  std::vector<cdouble> M(mat, mat + (size_t)n * n);
+  auto at = [&](int r, int c) -> cdouble& { return M[(size_t)r * n + c]; };
+  // Reck et al., PRL 73, 58 (1994): null each row from the bottom, left to
+  // right, by column rotations that push its weight into the diagonal
+  // column.  Every rotation is a right factor, so U = D G_N^* ... G_1^*
+  // with no rotation moved through the diagonal.
+  for (int row = n - 1; row >= 1; --row) {
+    for (int col = 0; col < row; ++col) {
+      if (std::abs(at(row, col)) <= tol) continue;
+      double c;
+      cdouble s;
+      zrotg_tol(at(row, col + 1), at(row, col), tol, c, s);
+      out.rotations.push_back({c, s, col + 1, col});
+      for (int r = 0; r < n; ++r) {
+        const cdouble xo = at(r, col + 1), yo = at(r, col);
+        at(r, col + 1) = c * xo + s * yo;
+        at(r, col) = c * yo - std::conj(s) * xo;
+      }
+    }
+  }
+  out.diag.resize(n);
+  for (int k = 0; k < n; ++k) out.diag[k] = at(k, k);
+  return out;
+}
+
 KStep kernel_step_from_rotation(const GivensRotation& g) {
   KStep k{};
   cdouble sigma;

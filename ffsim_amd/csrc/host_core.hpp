@@ -109,6 +109,12 @@ struct GivensDecomposition {
 };
 // mat is row-major n x n.
 GivensDecomposition givens_decomposition(const cdouble* mat, int n, double tol);
+// The same factorization form U = D G_N^* ... G_1^* in triangle (Reck) order:
+// rows are nulled from the bottom, each by a left-to-right sweep of column
+// rotations.  The fused passes run this order: at 12-orbital windows it needs
+// 3 passes up to norb = 18 (Clements needs 4 from norb = 17) and at most as
+// many register sub-passes.
+GivensDecomposition triangle_decomposition(const cdouble* mat, int n, double tol);
 
 // ----------------------------------------------------- fused-pass planning
 // A rotation in kernel form: on the local pair x = (p occupied, p+1 empty),

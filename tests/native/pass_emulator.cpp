@@ -54,9 +54,13 @@ static void run_pass(const C* src, C* dst, long ld_src, long ld_dst, int n_batch
   }
 }
 
-static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* mat, int w, C* tdst, long ld_t, int merge) {
+static GivensDecomposition decompose_in(const C* mat, int norb, int clements) {
+  return clements ? givens_decomposition(mat, norb, 1e-12) : triangle_decomposition(mat, norb, 1e-12);
+}
+
+static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* mat, int w, C* tdst, long ld_t, int merge, int clements) {
   std::vector<KStep> steps; std::vector<C> diag(norb, 1.0); bool has_diag = false;
-  if (mat) { auto g = givens_decomposition(mat, norb, 1e-12); for (auto& r : g.rotations) steps.push_back(kernel_step_from_rotation(r)); diag = g.diag; has_diag = true; }
+  if (mat) { auto g = decompose_in(mat, norb, clements); for (auto& r : g.rotations) steps.push_back(kernel_step_from_rotation(r)); diag = g.diag; has_diag = true; }
   auto passes = plan_passes(steps, norb, w, tdst != nullptr, -1);
   auto strings = make_strings(norb, nocc);
   auto GT = make_group_tables(w, kMaxSubWindow, merge != 0);
@@ -67,9 +71,38 @@ static void sector(C* state, long ld, int n_batch, int norb, int nocc, const C* 
   }
 }
 
-extern "C" void emulate(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w, int merge) {
+// clements = 0: the triangle (Reck) order; 1: ffsim's (Clements) order.  The library
+// picks one per norb and window width, by the window passes and layer pairs
+// that each needs.
+extern "C" void emulate_order(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w, int merge, int clements) {
   long da = binom(norb, na), db = binom(norb, nb);
   C* v = (C*)vec; C* wk = (C*)work;
-  sector(v, db, (int)db, norb, na, (const C*)ma, w, wk, da, merge);  // alpha rows, tstore into work
-  sector(wk, da, (int)da, norb, nb, (const C*)mb, w, v, db, merge);  // beta rows on transposed, tstore back
+  sector(v, db, (int)db, norb, na, (const C*)ma, w, wk, da, merge, clements);  // alpha rows, tstore into work
+  sector(wk, da, (int)da, norb, nb, (const C*)mb, w, v, db, merge, clements);  // beta rows on transposed, tstore back
+}
+
+extern "C" void emulate(double* vec, double* work, int norb, int na, int nb, const double* ma, const double* mb, int w, int merge) {
+  emulate_order(vec, work, norb, na, nb, ma, mb, w, merge, 0);
+}
+
+// The decomposition the passes run: rotations (c, s, i, j) in application
+// order and the diagonal; returns the rotation count (at most n(n-1)/2).
+extern "C" int decompose(const double* mat, int n, int clements, double* c, double* s, int* i, int* j, double* diag) {
+  auto g = decompose_in((const C*)mat, n, clements);
+  for (size_t k = 0; k < g.rotations.size(); ++k) {
+    c[k] = g.rotations[k].c; s[2 * k] = g.rotations[k].s.real(); s[2 * k + 1] = g.rotations[k].s.imag();
+    i[k] = g.rotations[k].i; j[k] = g.rotations[k].j;
+  }
+  for (int k = 0; k < n; ++k) { diag[2 * k] = g.diag[k].real(); diag[2 * k + 1] = g.diag[k].imag(); }
+  return (int)g.rotations.size();
+}
+
+// Window passes and register sub-passes the planner makes for one sector
+// rotation by mat at window width w (with the transposing last pass).
+extern "C" void plan_counts(const double* mat, int n, int w, int clements, int* n_passes, int* n_sub) {
+  auto g = decompose_in((const C*)mat, n, clements);
+  std::vector<KStep> steps; for (auto& r : g.rotations) steps.push_back(kernel_step_from_rotation(r));
+  auto passes = plan_passes(steps, n, w, true, -1);
+  *n_passes = (int)passes.size(); *n_sub = 0;
+  for (auto& P : passes) { std::vector<SubPass> subs; std::vector<LayerPair> lps; plan_sub_passes(P.steps, w, kMaxSubWindow, subs, lps); *n_sub += (int)subs.size(); }
 }

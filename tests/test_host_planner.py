@@ -36,7 +36,15 @@ def emulator(tmp_path_factory):
     P = ctypes.c_void_p
     lib.emulate.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_int,
                             ctypes.c_int]
+    lib.emulate_order.argtypes = lib.emulate.argtypes + [ctypes.c_int]
+    lib.decompose.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]
+    lib.decompose.restype = ctypes.c_int
+    lib.plan_counts.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
     return lib
+
+
+def _p(a):
+    return np.ascontiguousarray(a).ctypes.data_as(ctypes.c_void_p)
 
 
 @pytest.mark.parametrize("merge", [0, 1])
@@ -57,3 +65,60 @@ def test_emulated_passes_match_oracle(emulator, norb, nelec, w, merge, rng):
     ua, ub = np.ascontiguousarray(Ua), np.ascontiguousarray(Ub)
     emulator.emulate(p(x), p(work), norb, nelec[0], nelec[1], p(ua), p(ub), w, merge)
     np.testing.assert_allclose(x, expected, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize("norb,nelec,w", [(8, (4, 4), 5), (10, (5, 4), 6), (13, (6, 5), 12)])
+def test_emulated_passes_in_clements_order(emulator, norb, nelec, w, rng):
+    """FFSIM_AMD_GIVENS_ORDER=clements: ffsim's rotation order through the same passes."""
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    vec = O.random_state_vector(da * db, rng)
+    Ua, Ub = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+    expected = O.apply_orbital_rotation(vec, (Ua, Ub), norb, nelec)
+    x, work = vec.copy(), np.zeros_like(vec)
+    emulator.emulate_order(_p(x), _p(work), norb, nelec[0], nelec[1], _p(Ua), _p(Ub), w, 0, 1)
+    np.testing.assert_allclose(x, expected, atol=1e-12, rtol=0)
+
+
+def _decompose(emulator, U, clements):
+    n = U.shape[0]
+    m = n * (n - 1) // 2
+    c, s = np.zeros(m), np.zeros(2 * m)
+    i, j = np.zeros(m, dtype=np.int32), np.zeros(m, dtype=np.int32)
+    d = np.zeros(2 * n)
+    k = emulator.decompose(_p(U), n, clements, _p(c), _p(s), _p(i), _p(j), _p(d))
+    return c[:k], (s[0::2] + 1j * s[1::2])[:k], i[:k], j[:k], d[0::2] + 1j * d[1::2]
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 16, 18])
+def test_triangle_decomposition_reconstructs(emulator, n, rng):
+    """U = D G_N^* ... G_1^*, G = [[c, s], [-conj(s), c]] on adjacent (i, j):
+    the same form as ffsim.linalg.givens_decomposition, rotations in Reck's
+    order (each row nulled left to right, rows from the bottom)."""
+    U = O.random_unitary(n, rng)
+    c, s, i, j, d = _decompose(emulator, U, 0)
+    assert len(c) == n * (n - 1) // 2
+    assert np.all(np.abs(i - j) == 1)
+    np.testing.assert_allclose(np.abs(d), 1.0, atol=1e-12)
+    acc = np.diag(d)
+    for ck, sk, ik, jk in zip(c[::-1], s[::-1], i[::-1], j[::-1]):
+        G = np.eye(n, dtype=complex)
+        G[ik, ik], G[ik, jk], G[jk, ik], G[jk, jk] = ck, sk, -np.conj(sk), ck
+        acc = acc @ G.conj()
+    np.testing.assert_allclose(acc, U, atol=1e-12)
+    # the first sweep walks row n-1 from orbital pair (0, 1) to (n-2, n-1)
+    np.testing.assert_array_equal(np.minimum(i, j)[: n - 1], np.arange(n - 1))
+
+
+def test_triangle_order_needs_fewer_passes(emulator, rng):
+    """At 12-orbital windows the triangle order needs 3 passes at norb 17 and 18
+    (ffsim's order 4) and no more register sub-passes anywhere."""
+    for n in range(13, 21):
+        U = O.random_unitary(n, rng)
+        got = {}
+        for clements in (0, 1):
+            np_, ns = ctypes.c_int(), ctypes.c_int()
+            emulator.plan_counts(_p(U), n, 12, clements, ctypes.byref(np_), ctypes.byref(ns))
+            got[clements] = (np_.value, ns.value)
+        assert got[0][0] <= got[1][0] and got[0][1] <= got[1][1], (n, got)
+        if n in (17, 18):
+            assert got[0][0] == 3 and got[1][0] == 4, (n, got)

@@ -728,6 +728,8 @@ def main() -> int:
                     help="skip the c64-vs-c128 comparison (keeps profiler runs to one dtype)")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--only", choices=["gates", "trotter", "batched", "serving"], default=None,
+                    help="after the headline run, run this one leg and no other (for profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other-dtype LUCJ rate, the plain gates and the numpy-io legs")
     ap.add_argument("--serving-requests", type=int, default=400)
@@ -863,6 +865,16 @@ def main() -> int:
     vec_host = None if args.no_cpu_baseline else state.to(torch.complex128).cpu().numpy()
     del state
     torch.cuda.empty_cache()
+    if args.only:
+        args.no_extras = args.no_trotter = args.no_gradient = args.no_serving = True
+        if args.only == "gates":
+            result["gates_norb16"] = gates_leg(norb)
+        elif args.only == "trotter":
+            result["trotter_norb18"] = trotter_leg()
+        elif args.only == "batched":
+            result["batched_lucj_paths"] = batched_paths_leg()
+        else:
+            result["serving"] = serving_leg(args.serving_requests, args.serving_load)
     if not args.no_extras:
         other = torch.complex64 if args.dtype == "c128" else torch.complex128
         tag = "c64" if args.dtype == "c128" else "c128"

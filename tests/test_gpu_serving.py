@@ -104,6 +104,30 @@ def test_results_identical_alone_and_in_mix(rng):
         np.testing.assert_allclose(t2.result(), t.result(), atol=tol, rtol=0)
 
 
+def test_batched_serving_is_bitwise_alone_on_the_givens_path(rng, monkeypatch):
+    """With every batch on the Givens path, a request served in a batch gives
+    exactly the bits it gives alone."""
+    import torch
+
+    from ffsim_amd.serving import LUCJServer
+
+    monkeypatch.setenv("FFSIM_AMD_BATCH_PATH", "sweep")
+    _case("serving bitwise sweep")
+    reqs = _mixed_requests(rng, 12, SHAPES[:4])
+    alone = []
+    for r in reqs:
+        cdt = torch.complex128 if r.dtype == "c128" else torch.complex64
+        t = torch.from_numpy(r.state).to(cdt).cuda()
+        r.op.apply_in_place(t, r.norb, r.nelec)
+        alone.append(t.cpu().numpy())
+    server = LUCJServer(hbm_budget=64 << 20, slots_per_stream=2, n_streams=2, max_batch=4)
+    tickets = [server.submit(r) for r in reqs]
+    server.drain(timeout=120)
+    assert server.stats()["failed"] == 0
+    for t, a in zip(tickets, alone):
+        np.testing.assert_array_equal(t.result(), a)
+
+
 def test_device_resident_requests_stay_on_the_device(rng):
     import torch
 

@@ -321,6 +321,7 @@ def gates_leg(norb: int = 16, reps: int = 5, seed: int = 21) -> dict:
                                                              copy=False), reps),
         }
         out[tag]["native_calls"] = _native_calls(psi.view(da, db), norb, nelec[0], rng, reps)
+        out[tag]["one_body_contraction"] = _one_body(psi, norb, nelec, rng, reps)
         log(f"gates {tag}: {out[tag]}")
         del psi
         torch.cuda.empty_cache()
@@ -381,6 +382,28 @@ def _native_calls(vec, norb: int, nocc: int, rng, reps: int) -> dict:
                                   "GBs": round(2 * layer_rows * row_bytes / t_singles / 1e9, 1)},
         "layer_one_call": {"ms": round(t_layer * 1e3, 3), "rotations": len(layer)},
     }
+
+
+def _one_body(psi, norb: int, nelec, rng, reps: int) -> dict:
+    """contract_one_body (both sectors, out = M^ psi) with a random complex M on
+    the resident state: wall ms per call and the state passes' rate (psi read
+    twice, out written, then read and written: 5 state sizes per call)."""
+    import torch
+
+    from ffsim_amd.contract.one_body import contract_one_body_into
+
+    mat = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    out = torch.empty_like(psi)
+    contract_one_body_into(psi, out, mat, norb, nelec)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        contract_one_body_into(psi, out, mat, norb, nelec)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    del out
+    return {"ms_per_call": round(t * 1e3, 3),
+            "state_pass_GBs": round(5 * psi.numel() * psi.element_size() / t / 1e9, 1)}
 
 
 def numpy_io_leg(op, norb: int, nelec, reps: int = 3, seed: int = 0) -> dict:
@@ -728,7 +751,8 @@ def main() -> int:
                     help="skip the c64-vs-c128 comparison (keeps profiler runs to one dtype)")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--only", choices=["gates", "trotter", "batched", "serving"], default=None,
+    ap.add_argument("--only", choices=["gates", "trotter", "gradient", "batched", "serving"],
+                    default=None,
                     help="after the headline run, run this one leg and no other (for profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other-dtype LUCJ rate, the plain gates and the numpy-io legs")
@@ -871,6 +895,8 @@ def main() -> int:
             result["gates_norb16"] = gates_leg(norb)
         elif args.only == "trotter":
             result["trotter_norb18"] = trotter_leg()
+        elif args.only == "gradient":
+            result["gradient_norb12"] = gradient_leg()
         elif args.only == "batched":
             result["batched_lucj_paths"] = batched_paths_leg()
         else:

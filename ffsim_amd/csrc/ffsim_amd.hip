@@ -1768,26 +1768,65 @@ int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, 
     for (int c = 0; c <= norb; ++c)
       for (int t = 0; t <= norb; ++t) tab[(size_t)c * (norb + 1) + t] = binom(c, t);
     const size_t toff = prog.append(tab.data(), tab.size() * sizeof(uint64_t));
+    // per-(pair, string) dot products of a chunk of strings, then the beta
+    // sector's lam^T and psi^T when they fit the call's scratch (its strings
+    // then index rows, as the alpha sector's do, and every read is coalesced)
+    const long n2 = 2l * norb * norb;
+    const long n_str_max = std::max(da, db);
+    const long chunk = std::min<long>(n_str_max, std::max<long>(4, (64l << 20) / (n2 * 8)));
+    size_t poff = 0;
+    require(prog.reserve((size_t)(n2 * chunk * 8), &poff), FFSIM_ERR_VALUE,
+            "transition RDM: scratch too large");
+    const size_t state_bytes = (size_t)(da * db) * elem_bytes(dtype);
+    size_t tpoff = 0;
+    const bool transpose_b = prog.reserve(2 * state_bytes, &tpoff);
     prog.commit();
     const DevStrings& sa = dev_strings(ctx, norb, n_alpha);
     const DevStrings& sb = dev_strings(ctx, norb, n_beta);
-    double* rdm_a = out;
-    double* rdm_b = out + 2l * norb * norb;
-    if (dtype == FFSIM_C128) {
-      hipLaunchKernelGGL(trdm_kernel<double>, dim3((unsigned)da), dim3(256), 0, st,
-                         (const Cx<double>*)lam, (const Cx<double>*)psi, (int)da, (int)db, norb, 0,
-                         sa.d, prog.dev<uint64_t>(toff), rdm_a);
-      hipLaunchKernelGGL(trdm_kernel<double>, dim3((unsigned)db), dim3(256), 0, st,
-                         (const Cx<double>*)lam, (const Cx<double>*)psi, (int)da, (int)db, norb, 1,
-                         sb.d, prog.dev<uint64_t>(toff), rdm_b);
-    } else {
-      hipLaunchKernelGGL(trdm_kernel<float>, dim3((unsigned)da), dim3(256), 0, st,
-                         (const Cx<float>*)lam, (const Cx<float>*)psi, (int)da, (int)db, norb, 0,
-                         sa.d, prog.dev<uint64_t>(toff), rdm_a);
-      hipLaunchKernelGGL(trdm_kernel<float>, dim3((unsigned)db), dim3(256), 0, st,
-                         (const Cx<float>*)lam, (const Cx<float>*)psi, (int)da, (int)db, norb, 1,
-                         sb.d, prog.dev<uint64_t>(toff), rdm_b);
-    }
+    double* part = prog.scratch<double>(poff);
+    const uint64_t* dtab = prog.dev<uint64_t>(toff);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      const Cx<T>* L = (const Cx<T>*)lam;
+      const Cx<T>* P = (const Cx<T>*)psi;
+      // one sector: strings n_str on rows (cols = false) or on columns
+      auto sector = [&](const Cx<T>* l, const Cx<T>* x, bool cols, long n_str, long n_other,
+                        const uint64_t* strs, double* rdm) {
+        for (long s0 = 0; s0 < n_str; s0 += chunk) {
+          const long cnt = std::min(chunk, n_str - s0);
+          const long nz = n2 * cnt;
+          hipLaunchKernelGGL(fill_zero_kernel,
+                             dim3((unsigned)std::min<long>((nz + 255) / 256, 65536)), dim3(256), 0,
+                             st, part, nz);
+          const dim3 grid((unsigned)((cnt + 3) / 4));
+          if (cols)
+            hipLaunchKernelGGL((trdm_links_kernel<T, true>), grid, dim3(256), 0, st, l, x,
+                               (int)n_str, (int)n_other, norb, strs, dtab, (int)s0, (int)cnt,
+                               part);
+          else
+            hipLaunchKernelGGL((trdm_links_kernel<T, false>), grid, dim3(256), 0, st, l, x,
+                               (int)n_str, (int)n_other, norb, strs, dtab, (int)s0, (int)cnt,
+                               part);
+          hipLaunchKernelGGL(trdm_reduce_kernel, dim3((unsigned)n2), dim3(256), 0, st, part,
+                             (int)cnt, rdm);
+        }
+      };
+      sector(L, P, false, da, db, sa.d, out);
+      if (transpose_b) {
+        Cx<T>* lt = prog.scratch<Cx<T>>(tpoff);
+        Cx<T>* pt = lt + da * db;
+        const dim3 tg((unsigned)((db + 31) / 32), (unsigned)((da + 31) / 32));
+        hipLaunchKernelGGL(transpose_kernel<T>, tg, dim3(256), 0, st, L, lt, (int)da, (int)db);
+        hipLaunchKernelGGL(transpose_kernel<T>, tg, dim3(256), 0, st, P, pt, (int)da, (int)db);
+        sector(lt, pt, false, db, da, sb.d, out + n2);
+      } else {
+        sector(L, P, true, db, da, sb.d, out + n2);
+      }
+    };
+    if (dtype == FFSIM_C128)
+      run(double{});
+    else
+      run(float{});
     HIP_CHECK(hipGetLastError());
     prog.finish();
   });

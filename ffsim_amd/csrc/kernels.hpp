@@ -1011,17 +1011,31 @@ __device__ __forceinline__ uint64_t colex_rank_dev(uint64_t s, const uint64_t* b
   return r;
 }
 
-template <typename T>
+// Transition one-body RDM of one spin sector, one wave per string S of the
+// sector: for q in S and p = q or p not in S, T = S - q + p and
+//   part[(p norb + q) 2 + c][S - s_begin] = sign * sum_o conj(lam(T, o)) psi(S, o)
+// (c: re, im) for the strings s_begin <= S < s_begin + n_chunk, where (S, o)
+// addresses row S of an n_str x n_other row-major matrix (COLS = false) or
+// column S of an n_other x n_str one (COLS = true).  The links of S run in a
+// wave-uniform loop, each dot product is reduced across the wave, and every
+// (pair, string) entry is written by exactly one wave, so trdm_reduce_kernel
+// sums them in a fixed order (deterministic results).  part must be zero where
+// S has no link (p, q).
+template <typename T, bool COLS>
 __global__ void __launch_bounds__(256)
-    trdm_kernel(const Cx<T>* __restrict__ lam, const Cx<T>* __restrict__ psi, int n_rows,
-                int n_cols, int norb, int sector, const uint64_t* __restrict__ strings,
-                const uint64_t* __restrict__ binom_tab, double* __restrict__ rdm) {
-  __shared__ double red_re[256], red_im[256];
-  const int s_idx = blockIdx.x;  // string of this sector
-  const int tid = threadIdx.x;
+    trdm_links_kernel(const Cx<T>* __restrict__ lam, const Cx<T>* __restrict__ psi, int n_str,
+                      int n_other, int norb, const uint64_t* __restrict__ strings,
+                      const uint64_t* __restrict__ binom_tab, int s_begin, int n_chunk,
+                      double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int s_loc = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int s_idx = s_begin + s_loc;
+  if (s_loc >= n_chunk || s_idx >= n_str) return;
   const uint64_t S = strings[s_idx];
-  const int n_other = (sector == 0) ? n_cols : n_rows;
   const int stride = norb + 1;
+  auto at = [&](const Cx<T>* m, long str, int o) -> Cx<T> {
+    return COLS ? m[(long)o * n_str + str] : m[str * n_other + o];
+  };
   uint64_t qbits = S;
   while (qbits) {
     const int q = __builtin_ctzll(qbits);
@@ -1035,35 +1049,63 @@ __global__ void __launch_bounds__(256)
       const double sgn = ((sign_q ^ sign_p) & 1) ? -1.0 : 1.0;
       const long t_idx = (long)colex_rank_dev(S1, binom_tab, stride);
       double re = 0.0, im = 0.0;
-      for (int o = tid; o < n_other; o += 256) {
-        Cx<T> l, x;
-        if (sector == 0) {
-          l = lam[t_idx * n_cols + o];
-          x = psi[(long)s_idx * n_cols + o];
-        } else {
-          l = lam[(long)o * n_cols + t_idx];
-          x = psi[(long)o * n_cols + s_idx];
-        }
+#pragma unroll 4
+      for (int o = lane; o < n_other; o += 64) {
+        const Cx<T> l = at(lam, t_idx, o), x = at(psi, s_idx, o);
         // conj(l) * x
         re += (double)l.re * (double)x.re + (double)l.im * (double)x.im;
         im += (double)l.re * (double)x.im - (double)l.im * (double)x.re;
       }
-      red_re[tid] = re;
-      red_im[tid] = im;
-      __syncthreads();
-      for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-          red_re[tid] += red_re[tid + s];
-          red_im[tid] += red_im[tid + s];
-        }
-        __syncthreads();
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
       }
-      if (tid == 0) {
-        atomicAdd(&rdm[2 * (p * norb + q)], sgn * red_re[0]);
-        atomicAdd(&rdm[2 * (p * norb + q) + 1], sgn * red_im[0]);
+      if (lane == 0) {
+        const long e = 2l * (p * norb + q);
+        part[e * n_chunk + s_loc] = sgn * re;
+        part[(e + 1) * n_chunk + s_loc] = sgn * im;
       }
-      __syncthreads();
     }
+  }
+}
+
+// rdm[e] += sum over the chunk's strings s of part[e][s], one block per entry
+// e, in a fixed order.
+static __global__ void __launch_bounds__(256)
+    trdm_reduce_kernel(const double* __restrict__ part, int n_chunk, double* __restrict__ rdm) {
+  __shared__ double red[256];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  const double* row = part + (long)e * n_chunk;
+  double acc = 0.0;
+  for (int s = tid; s < n_chunk; s += 256) acc += row[s];
+  red[tid] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) rdm[e] += red[0];
+}
+
+// out (cols x rows) = in^T for a row-major rows x cols matrix, 32 x 32 tiles
+// through LDS (padded against bank conflicts).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    transpose_kernel(const Cx<T>* __restrict__ in, Cx<T>* __restrict__ out, int rows, int cols) {
+  __shared__ Cx<T> tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int r = r0 + ty + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = in[(long)r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int c = c0 + ty + k, r = r0 + tx;
+    if (r < rows && c < cols) out[(long)c * rows + r] = tile[tx][ty + k];
   }
 }
 

@@ -69,6 +69,26 @@ def test_transition_rdm_and_phase_reductions(norb, nelec, dtype, rng):
     np.testing.assert_allclose(M, e_M, atol=tol, rtol=0)
 
 
+@pytest.mark.parametrize("norb, nelec", [(14, (7, 6)), (20, (5, 1))])
+def test_transition_rdm_large_sectors(norb, nelec, rng):
+    """(14, (7, 6)): the beta sector's transposes (2 x 165 MB) exceed the call's
+    scratch, so its strings are read along columns in place; (20, (5, 1)): the
+    15504 alpha strings are reduced in several chunks."""
+    torch = _torch()
+    from ffsim_amd.variational.gradient import DeviceAdjointOps
+
+    _case("trdm large", norb, nelec)
+    dim = _dim(norb, nelec)
+    lam, psi = O.random_state_vector(dim, rng), O.random_state_vector(dim, rng)
+    ops = DeviceAdjointOps(norb, nelec)
+    lam_d, psi_d = torch.from_numpy(lam).cuda(), torch.from_numpy(psi).cuda()
+    rdm = ops.host_rdm(ops.transition_rdm(lam_d, psi_d))
+    np.testing.assert_allclose(rdm, O.transition_rdm(lam, psi, norb, nelec), atol=1e-12, rtol=0)
+    # the reduction order is fixed: a second call gives the same bits
+    again = ops.host_rdm(ops.transition_rdm(lam_d, psi_d))
+    np.testing.assert_array_equal(rdm, again)
+
+
 def test_autograd_matches_oracle_finite_difference(rng):
     """loss.backward() through ucj_state and expectation, small case, all parameters."""
     torch = _torch()

@@ -1730,22 +1730,37 @@ int ffsim_diag_coulomb_grad_reduce(const void* psi, const void* lam, int dtype, 
     double* rsum = out;
     double* csum = out + da;
     double* M = out + da + db;
+    // column sums over row chunks (enough blocks to fill the device), then the
+    // chunks of each column added in order
+    const long col_blocks = (db + 255) / 256;
+    const long n_chunks = std::max(1l, std::min<long>(da, (4l * g_n_cu + col_blocks - 1) / col_blocks));
+    const long rows_per_chunk = (da + n_chunks - 1) / n_chunks;
+    Program prog(ctx, st);
+    size_t poff = 0;
+    require(prog.reserve((size_t)(n_chunks * db) * sizeof(double), &poff), FFSIM_ERR_VALUE,
+            "gradient reductions: scratch too large");
+    prog.commit();
+    double* part = prog.scratch<double>(poff);
+    const dim3 cgrid((unsigned)col_blocks, (unsigned)n_chunks);
     if (dtype == FFSIM_C128) {
       hipLaunchKernelGGL(grad_rows_kernel<double>, dim3((unsigned)da), dim3(256), 0, st,
                          (const Cx<double>*)psi, (const Cx<double>*)lam, (int)da, (int)db, norb,
                          cs.d, rsum, M);
-      hipLaunchKernelGGL(grad_cols_kernel<double>, dim3((unsigned)((db + 255) / 256)), dim3(256),
-                         0, st, (const Cx<double>*)psi, (const Cx<double>*)lam, (int)da, (int)db,
-                         csum);
+      hipLaunchKernelGGL(grad_cols_kernel<double>, cgrid, dim3(256), 0, st,
+                         (const Cx<double>*)psi, (const Cx<double>*)lam, (int)da, (int)db,
+                         (int)rows_per_chunk, part);
     } else {
       hipLaunchKernelGGL(grad_rows_kernel<float>, dim3((unsigned)da), dim3(256), 0, st,
                          (const Cx<float>*)psi, (const Cx<float>*)lam, (int)da, (int)db, norb,
                          cs.d, rsum, M);
-      hipLaunchKernelGGL(grad_cols_kernel<float>, dim3((unsigned)((db + 255) / 256)), dim3(256),
-                         0, st, (const Cx<float>*)psi, (const Cx<float>*)lam, (int)da, (int)db,
-                         csum);
+      hipLaunchKernelGGL(grad_cols_kernel<float>, cgrid, dim3(256), 0, st,
+                         (const Cx<float>*)psi, (const Cx<float>*)lam, (int)da, (int)db,
+                         (int)rows_per_chunk, part);
     }
+    hipLaunchKernelGGL(col_chunk_sum_kernel, dim3((unsigned)col_blocks), dim3(256), 0, st, part,
+                       (int)n_chunks, (int)db, csum);
     HIP_CHECK(hipGetLastError());
+    prog.finish();
   });
 }
 

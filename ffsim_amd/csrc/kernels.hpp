@@ -941,7 +941,8 @@ __global__ void __launch_bounds__(256)
 //
 // grad_rows_kernel: for one row a per block, with w(a,b) = Im(conj(lam) psi):
 //   rsum[a] = sum_b w,   M[a][j] = sum_{b : j in b} w   (j < norb <= 32).
-// grad_cols_kernel: csum[b] = sum_a w (one column per thread).
+// grad_cols_kernel: per chunk of rows, the column sums of w (one column per
+// thread, grid.y = row chunk); col_chunk_sum_kernel adds the chunks.
 template <typename T>
 __global__ void __launch_bounds__(256)
     grad_rows_kernel(const Cx<T>* __restrict__ psi, const Cx<T>* __restrict__ lam, int n_rows,
@@ -980,15 +981,28 @@ __global__ void __launch_bounds__(256)
 }
 
 template <typename T>
-__global__ void grad_cols_kernel(const Cx<T>* __restrict__ psi, const Cx<T>* __restrict__ lam,
-                                 int n_rows, int n_cols, double* __restrict__ csum) {
+__global__ void __launch_bounds__(256)
+    grad_cols_kernel(const Cx<T>* __restrict__ psi, const Cx<T>* __restrict__ lam, int n_rows,
+                     int n_cols, int rows_per_chunk, double* __restrict__ part) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cols) return;
+  const int a0 = blockIdx.y * rows_per_chunk, a1 = min(n_rows, a0 + rows_per_chunk);
   double s = 0.0;
-  for (int a = 0; a < n_rows; ++a) {
+  for (int a = a0; a < a1; ++a) {
     const Cx<T> p = psi[(long)a * n_cols + c], l = lam[(long)a * n_cols + c];
     s += (double)l.re * (double)p.im - (double)l.im * (double)p.re;
   }
+  part[(long)blockIdx.y * n_cols + c] = s;
+}
+
+// csum[c] = sum over the row chunks k (in order) of part[k][c].
+static __global__ void __launch_bounds__(256)
+    col_chunk_sum_kernel(const double* __restrict__ part, int n_chunks, int n_cols,
+                         double* __restrict__ csum) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cols) return;
+  double s = 0.0;
+  for (int k = 0; k < n_chunks; ++k) s += part[(long)k * n_cols + c];
   csum[c] = s;
 }
 

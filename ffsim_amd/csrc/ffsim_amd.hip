@@ -205,6 +205,10 @@ struct Context {
   std::map<std::tuple<int, int, int, int>, DevWindow> windows;
   std::map<std::pair<int, bool>, DevGroups> groups;  // (window width, merged complements)
   std::map<std::tuple<int, int, int, int>, DevPairs> pairs;  // (norb, nocc, i, j)
+  // one-body window passes: fibers of (norb, nocc, window mask, between mask)
+  // and the link tables of each window width
+  std::map<std::tuple<int, int, uint64_t, uint64_t>, DevWindow> mask_windows;
+  std::map<int, std::pair<uint32_t*, int32_t*>> ob_links;
   ProgramRing ring;
 };
 
@@ -326,6 +330,36 @@ const DevLinks& dev_links(Context& ctx, int norb, int nocc) {
   dl.d = upload_table(tab);
   std::lock_guard<std::mutex> lk(ctx.mu);
   return ctx.links.emplace(std::make_pair(norb, nocc), dl).first->second;
+}
+
+const DevWindow& dev_mask_window(Context& ctx, int norb, int nocc, uint64_t mask,
+                                 uint64_t between) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  auto key = std::make_tuple(norb, nocc, mask, between);
+  auto it = ctx.mask_windows.find(key);
+  if (it != ctx.mask_windows.end()) return it->second;
+  WindowLayout L = make_mask_layout(norb, nocc, mask, between);
+  DevWindow dw;
+  dw.fibers = upload_table(L.fibers);
+  dw.rows = upload_table(L.row_table);
+  dw.n_fibers = (int)L.fibers.size();
+  dw.n_active = L.n_active;
+  dw.max_rows = L.max_rows;
+  dw.max_rows_active = L.max_rows_active;
+  dw.w = L.w;
+  dw.rows_all = (long)L.row_table.size();
+  dw.rows_active = dw.rows_all;
+  return ctx.mask_windows.emplace(key, dw).first->second;
+}
+
+std::pair<uint32_t*, int32_t*> dev_one_body_links(Context& ctx, int w) {
+  std::lock_guard<std::mutex> lk(ctx.mu);
+  auto it = ctx.ob_links.find(w);
+  if (it != ctx.ob_links.end()) return it->second;
+  OneBodyLinks B = make_one_body_links(w);
+  std::pair<uint32_t*, int32_t*> d(upload_table(B.links), upload_table(B.index));
+  ctx.ob_links.emplace(w, d);
+  return d;
 }
 
 const DevWindow& dev_window(Context& ctx, int norb, int nocc, int lo, int w) {
@@ -1415,6 +1449,40 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
 
 using namespace ffsim_amd;
 
+// Tile width and workgroup of the one-body window passes: 64-byte batch
+// segments, two workgroups per CU.
+constexpr int kOneBodyThreads = 512;
+int one_body_tile_width(int dtype) { return dtype == FFSIM_C64 ? 8 : 4; }
+
+template <typename T>
+void launch_one_body_pass(int mode, const void* src, void* dst, long ld, int n_batch,
+                          const DevWindow& win, const uint32_t* links, const int32_t* lidx,
+                          const double* mloc, double scale, int acc, size_t lds_max,
+                          hipStream_t st) {
+  constexpr int W = sizeof(T) == 8 ? 4 : 8;
+  const size_t lds = align16((size_t)win.max_rows * W * sizeof(Cx<T>)) +
+                     2 * (size_t)win.w * win.w * sizeof(double);
+  require(lds <= lds_max && win.max_rows <= 1024, FFSIM_ERR_VALUE, "one-body tile exceeds LDS");
+  const long n_tiles = (long)win.n_fibers * ((n_batch + W - 1) / W);
+  auto kern = mode == MODE_ROWS ? one_body_fiber_kernel<T, W, MODE_ROWS, kOneBodyThreads>
+                                : one_body_fiber_kernel<T, W, MODE_COLS, kOneBodyThreads>;
+  static bool attr_set[2] = {false, false};
+  if (!attr_set[mode]) {
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_max));
+    attr_set[mode] = true;
+  }
+  int occ = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern,
+                                                         kOneBodyThreads, lds));
+  const long grid = std::min<long>(n_tiles, (long)std::max(1, occ) * g_n_cu);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kOneBodyThreads), lds, st,
+                     reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld,
+                     n_batch, win.fibers, win.n_fibers, win.rows, win.max_rows, links, lidx, mloc,
+                     win.w, scale, acc);
+  HIP_CHECK(hipGetLastError());
+}
+
 extern "C" {
 
 const char* ffsim_amd_last_error(void) { return g_last_error.c_str(); }
@@ -1604,8 +1672,8 @@ int ffsim_contract_num_op_sum(const void* vec, void* out, int dtype, int norb, i
 }
 
 int ffsim_contract_one_body(const void* vec, void* out, int dtype, int norb, int n_alpha,
-                             int n_beta, const double* mat, double scale, int accumulate,
-                             void* stream) {
+                            int n_beta, const double* mat, double scale, int accumulate,
+                            void* stream) {
   return guarded([&] {
     check_dtype(dtype);
     check_sizes(norb, n_alpha, n_beta);
@@ -1616,6 +1684,56 @@ int ffsim_contract_one_body(const void* vec, void* out, int dtype, int norb, int
     hipStream_t st = (hipStream_t)stream;
     if (da == 0 || db == 0) return;
     require(da < (1l << 31) && db < (1l << 31), FFSIM_ERR_VALUE, "dimension too large");
+    // window passes when every sector's windows fit a tile
+    const int elem = (int)elem_bytes(dtype), Wt = one_body_tile_width(dtype);
+    auto fits = [&](int nocc, int w) {
+      return (size_t)max_fiber(norb, nocc, std::min(w, norb)) * Wt * elem + 2 * w * w * 8 + 64 <=
+             ctx.lds_max / 2;
+    };
+    int w = std::min(norb, 12);
+    while (w > 1 && !(fits(n_alpha, w) && fits(n_beta, w))) --w;
+    const std::vector<OneBodyPass> passes = one_body_passes(norb, w);
+    if (!passes.empty() && norb >= 1 && norb <= 31) {
+      Program prog(ctx, st);
+      const cdouble* M = reinterpret_cast<const cdouble*>(mat);
+      std::vector<size_t> moffs;
+      for (const OneBodyPass& P : passes) {
+        std::vector<int> loc;
+        for (int b = 0; b < norb; ++b)
+          if ((P.mask >> b) & 1) loc.push_back(b);
+        const int wl = (int)loc.size();
+        std::vector<double> m((size_t)2 * wl * wl, 0.0);
+        for (int i = 0; i < wl; ++i)
+          for (int j = 0; j < wl; ++j)
+            if (P.take[(size_t)loc[i] * norb + loc[j]]) {
+              m[2 * (i * wl + j)] = M[(size_t)loc[i] * norb + loc[j]].real();
+              m[2 * (i * wl + j) + 1] = M[(size_t)loc[i] * norb + loc[j]].imag();
+            }
+        moffs.push_back(prog.append(m.data(), m.size() * sizeof(double)));
+      }
+      prog.commit();
+      int acc = accumulate;
+      for (int sector = 0; sector < 2; ++sector) {
+        const int nocc = sector == 0 ? n_alpha : n_beta;
+        const int mode = sector == 0 ? MODE_ROWS : MODE_COLS;
+        const long n_batch = sector == 0 ? db : da;
+        for (size_t k = 0; k < passes.size(); ++k) {
+          const DevWindow& win = dev_mask_window(ctx, norb, nocc, passes[k].mask, passes[k].between);
+          const auto lk = dev_one_body_links(ctx, win.w);
+          if (dtype == FFSIM_C128)
+            launch_one_body_pass<double>(mode, vec, out, db, (int)n_batch, win, lk.first,
+                                         lk.second, prog.dev<double>(moffs[k]), scale, acc,
+                                         ctx.lds_max, st);
+          else
+            launch_one_body_pass<float>(mode, vec, out, db, (int)n_batch, win, lk.first,
+                                        lk.second, prog.dev<double>(moffs[k]), scale, acc,
+                                        ctx.lds_max, st);
+          acc = 1;
+        }
+      }
+      prog.finish();
+      return;
+    }
     Program prog(ctx, st);
     const size_t moff = prog.append(mat, sizeof(double) * 2 * norb * norb);
     prog.commit();

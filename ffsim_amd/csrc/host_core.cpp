@@ -121,6 +121,108 @@ WindowLayout make_window_layout(int norb, int nocc, int lo, int w) {
   return L;
 }
 
+WindowLayout make_mask_layout(int norb, int nocc, uint64_t win_mask, uint64_t between_mask) {
+  const uint64_t all_mask = (norb == 64 ? ~0ull : ((1ull << norb) - 1));
+  if ((win_mask & ~all_mask) || !win_mask) throw std::invalid_argument("bad window mask");
+  WindowLayout L;
+  L.lo = 0;
+  L.w = __builtin_popcountll(win_mask);
+  L.contiguous = false;
+  const int w = L.w;
+  std::vector<int> in_pos, out_pos;
+  for (int b = 0; b < norb; ++b) ((win_mask >> b) & 1 ? in_pos : out_pos).push_back(b);
+  const int n_out = (int)out_pos.size();
+  auto deposit = [&](uint64_t loc) {
+    uint64_t s = 0;
+    for (int t = 0; t < w; ++t)
+      if ((loc >> t) & 1) s |= 1ull << in_pos[t];
+    return s;
+  };
+  for (uint64_t m = 0; m < (1ull << n_out); ++m) {
+    uint64_t fixed = 0;
+    for (int t = 0; t < n_out; ++t)
+      if ((m >> t) & 1) fixed |= 1ull << out_pos[t];
+    const int kp = nocc - __builtin_popcountll(fixed);
+    if (kp < 0 || kp > w) continue;
+    Fiber f;
+    f.row_off = (int)L.row_table.size();
+    f.kp = kp;
+    f.n_rows = (int)binom(w, kp);
+    f.pad = __builtin_popcountll(fixed & between_mask) & 1;
+    for (uint64_t loc : make_strings(w, kp))
+      L.row_table.push_back((int32_t)string_rank(fixed | deposit(loc)));
+    L.max_rows = std::max(L.max_rows, f.n_rows);
+    L.fibers.push_back(f);
+  }
+  L.n_active = (int)L.fibers.size();
+  L.max_rows_active = L.max_rows;
+  return L;
+}
+
+OneBodyLinks make_one_body_links(int w) {
+  if (w < 1 || w > 31) throw std::invalid_argument("one-body window width must be in [1, 31]");
+  OneBodyLinks B;
+  B.w = w;
+  B.index.assign((size_t)(w + 1) * 2, 0);
+  for (int kp = 0; kp <= w; ++kp) {
+    const size_t base = B.links.size();
+    const int nl = kp * (w - kp + 1);
+    const std::vector<uint64_t> locals = make_strings(w, kp);
+    const size_t n_rows = locals.size();
+    B.index[2 * kp] = (int32_t)base;
+    B.index[2 * kp + 1] = nl;
+    B.links.resize(base + n_rows * nl);
+    for (size_t i = 0; i < n_rows; ++i) {
+      const uint64_t t = locals[i];
+      int l = 0;
+      for (int p = 0; p < w; ++p) {
+        if (!((t >> p) & 1)) continue;
+        for (int q = 0; q < w; ++q) {
+          if (q != p && ((t >> q) & 1)) continue;
+          const uint64_t src = (t & ~(1ull << p)) | (1ull << q);
+          int neg = 0;
+          if (p != q) {
+            const int lo = std::min(p, q), hi = std::max(p, q);
+            const uint64_t between = ((1ull << hi) - 1) & ~((1ull << (lo + 1)) - 1);
+            neg = __builtin_popcountll(src & between) & 1;
+          }
+          B.links[base + (size_t)l * n_rows + i] = (uint32_t)string_rank(src) |
+                                                   ((uint32_t)p << 16) | ((uint32_t)q << 21) |
+                                                   ((uint32_t)neg << 26);
+          ++l;
+        }
+      }
+    }
+  }
+  return B;
+}
+
+std::vector<OneBodyPass> one_body_passes(int norb, int w) {
+  std::vector<OneBodyPass> out;
+  if (norb < 1 || norb > 62 || w < 1) return out;
+  const uint64_t all = (1ull << norb) - 1;
+  auto pass = [&](uint64_t mask, uint64_t between, auto take) {
+    OneBodyPass P;
+    P.mask = mask;
+    P.between = between;
+    P.take.assign((size_t)norb * norb, 0);
+    for (int p = 0; p < norb; ++p)
+      for (int q = 0; q < norb; ++q) P.take[(size_t)p * norb + q] = take(p, q) ? 1 : 0;
+    out.push_back(std::move(P));
+  };
+  if (norb <= w) {
+    pass(all, 0, [](int, int) { return true; });
+  } else if (2 * (norb - w) <= w) {
+    const int a = norb - w;  // A = [0, a), B = [a, w), C = [w, norb)
+    const uint64_t mA = (1ull << a) - 1, mB = ((1ull << w) - 1) & ~mA, mC = all & ~((1ull << w) - 1);
+    pass(mA | mB, 0, [&](int p, int q) { return p < w && q < w; });
+    pass(mB | mC, 0, [&](int p, int q) { return p >= a && q >= a && !(p < w && q < w); });
+    pass(mA | mC, mB, [&](int p, int q) { return (p < a && q >= w) || (p >= w && q < a); });
+  }
+  return out;
+}
+
+
 PairTables make_pair_tables(int w) {
   PairTables T;
   T.w = w;

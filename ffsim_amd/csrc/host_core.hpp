@@ -49,6 +49,42 @@ struct WindowLayout {
 };
 WindowLayout make_window_layout(int norb, int nocc, int lo, int w);
 
+// Fibers of a window given as an orbital mask (not necessarily contiguous):
+// strings grouped by their bits outside the mask, each fiber's strings in
+// ascending order of their bits inside it (window-local strings deposited into
+// the mask's orbitals in ascending order).  Every fiber is listed, kp = 0 and
+// kp = w included.  Fiber::pad holds the parity of the fiber's fixed (outside)
+// electrons on the orbitals of between_mask.
+WindowLayout make_mask_layout(int norb, int nocc, uint64_t win_mask, uint64_t between_mask);
+
+// One-body links of window-local strings (w orbitals, kp electrons): for each
+// local string T in ascending order, the kp (w - kp + 1) pairs (p, q) with p in
+// T and q = p or q not in T, and S = T - p + q, so that a+_p a_q |S> = sign |T>
+// with sign the parity of S's electrons strictly between p and q (local
+// orbitals).  Entries are packed S | p << 16 | q << 21 | (sign < 0) << 26;
+// index[2 kp] is the offset of the kp block and index[2 kp + 1] the links per
+// string; link l of string i is entry index[2 kp] + l * C(w, kp) + i (link
+// major, so that consecutive strings read consecutive words).  w <= 31.
+struct OneBodyLinks {
+  int w = 0;
+  std::vector<uint32_t> links;
+  std::vector<int32_t> index;
+};
+OneBodyLinks make_one_body_links(int w);
+
+// Window passes of the one-body contraction of one spin sector: with w the
+// widest window whose fibers fit a tile, norb <= w is one pass over all
+// orbitals; norb <= 1.5 w splits the orbitals into A = [0, norb - w),
+// B = [norb - w, w), C = [w, norb) and runs the windows A+B (pairs inside it),
+// B+C (pairs inside it and not inside A+B) and A+C (pairs between A and C, the
+// electrons on B giving each fiber's sign).  Every ordered pair (p, q) is
+// applied by exactly one pass.  Returns no passes for wider norb.
+struct OneBodyPass {
+  uint64_t mask = 0, between = 0;
+  std::vector<char> take;  // norb x norb: pair (p, q) applied by this pass
+};
+std::vector<OneBodyPass> one_body_passes(int norb, int w);
+
 // Local pair tables for a window of width w: for each kp in [0, w] and each
 // adjacent local pair (p, p + 1), the list of (x, y) local ranks with x having
 // p occupied / p + 1 empty and y = x with the electron moved to p + 1.

@@ -1167,6 +1167,109 @@ __global__ void __launch_bounds__(256)
   out[o] = Cx<T>{(T)r, (T)i};
 }
 
+// One-body contraction by window passes: out (+)= scale * sum_{p,q in the
+// window} m[p][q] a+_p a_q psi for one spin sector, where m is the pass's
+// window-local coefficient matrix (zero for the pairs another pass applies).
+// A workgroup stages one fiber x W batch vectors of psi in LDS; every string T
+// of the fiber then gathers its kp (w - kp + 1) sources S (links of T, see
+// make_one_body_links) from LDS, so psi is read once per pass.  Fiber::pad
+// negates the fiber's contributions: the parity of its fixed electrons between
+// the two parts of a split window.
+// MODE_ROWS: element (string r, batch b) at src[r * ld + b]; MODE_COLS: at
+// src[b * ld + r].  accumulate: 0 -> out = scale acc, 1 -> out += scale acc.
+template <typename T, int W, int MODE, int NTHR>
+__global__ void __launch_bounds__(NTHR)
+    one_body_fiber_kernel(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, long ld,
+                          int n_batch, const Fiber* __restrict__ fibers, int n_fibers,
+                          const int32_t* __restrict__ row_table, int max_rows,
+                          const uint32_t* __restrict__ links,
+                          const int32_t* __restrict__ link_index,
+                          const double* __restrict__ mloc, int w, double scale, int accumulate) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* tile = reinterpret_cast<Cx<T>*>(smem_raw);
+  double* mc = reinterpret_cast<double*>(smem_raw + align16((size_t)max_rows * W * sizeof(Cx<T>)));
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 2 * w * w; i += NTHR) mc[i] = mloc[i];
+  const int n_bt = (n_batch + W - 1) / W;
+  const long n_tiles = (long)n_fibers * n_bt;
+  // XCD-aware order (as in fiber_kernel): each XCD takes runs of consecutive
+  // tiles, so the lines that neighbouring batch tiles share hit in its L2
+  const int G = gridDim.x;
+  const long bid = (G & 7) == 0 ? (long)(blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3)
+                                : (long)blockIdx.x;
+  auto addr = [&](long r, long b) { return MODE == MODE_ROWS ? r * ld + b : b * ld + r; };
+  for (long t = bid; t < n_tiles; t += G) {
+    const Fiber fd = fibers[t / n_bt];
+    const int b0 = (int)(t % n_bt) * W;
+    const int nb = min(W, n_batch - b0);
+    const int nr = fd.n_rows;
+    const int32_t* rows = row_table + fd.row_off;
+    __syncthreads();  // the previous tile's store is done reading the LDS tile
+    for (int e = tid; e < nr * W; e += NTHR) {
+      const int i = MODE == MODE_ROWS ? e / W : e % nr;
+      const int j = MODE == MODE_ROWS ? e % W : e / nr;
+      tile[i * W + j] = j < nb ? src[addr(rows[i], b0 + j)] : Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    // one thread per string T: each link (one table word, one coefficient) is
+    // applied to all W batch columns of the tile.  Results go back into the
+    // tile once every thread is done reading it, and are stored with the
+    // load's access pattern (lanes along strings in MODE_COLS, so that a
+    // fiber's contiguous addresses are written together).
+    const int off = link_index[2 * fd.kp], nl = link_index[2 * fd.kp + 1];
+    const double fs = fd.pad ? -scale : scale;
+    constexpr int SPT = (1024 + NTHR - 1) / NTHR;  // strings per thread (fibers <= 1024)
+    double ar[SPT][W], ai[SPT][W];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int i = tid + k * NTHR;
+#pragma unroll
+      for (int j = 0; j < W; ++j) ar[k][j] = ai[k][j] = 0.0;
+      if (i >= nr) continue;
+      const uint32_t* lk = links + off + i;
+#pragma unroll 4
+      for (int l = 0; l < nl; ++l) {
+        const uint32_t u = lk[(long)l * nr];
+        const int S = (int)(u & 0xffff), pq = ((u >> 16) & 31) * w + ((u >> 21) & 31);
+        double mr = mc[2 * pq], mi = mc[2 * pq + 1];
+        if (u >> 26) {
+          mr = -mr;
+          mi = -mi;
+        }
+        const Cx<T>* xs = tile + S * W;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const Cx<T> x = xs[j];
+          ar[k][j] += mr * (double)x.re - mi * (double)x.im;
+          ai[k][j] += mr * (double)x.im + mi * (double)x.re;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int i = tid + k * NTHR;
+      if (i >= nr) continue;
+#pragma unroll
+      for (int j = 0; j < W; ++j) tile[i * W + j] = Cx<T>{(T)(fs * ar[k][j]), (T)(fs * ai[k][j])};
+    }
+    __syncthreads();
+    for (int e = tid; e < nr * W; e += NTHR) {
+      const int i = MODE == MODE_ROWS ? e / W : e % nr;
+      const int j = MODE == MODE_ROWS ? e % W : e / nr;
+      if (j >= nb) continue;
+      const long o = addr(rows[i], b0 + j);
+      Cx<T> y = tile[i * W + j];
+      if (accumulate) {
+        const Cx<T> z = dst[o];
+        y.re += z.re;
+        y.im += z.im;
+      }
+      dst[o] = y;
+    }
+  }
+}
+
 // x[0..n) = 0 (defined in ffsim_amd.hip).
 __global__ void fill_zero_kernel(double* __restrict__ x, long n);
 

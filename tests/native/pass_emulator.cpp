@@ -106,3 +106,45 @@ extern "C" void plan_counts(const double* mat, int n, int w, int clements, int* 
   *n_passes = (int)passes.size(); *n_sub = 0;
   for (auto& P : passes) { std::vector<SubPass> subs; std::vector<LayerPair> lps; plan_sub_passes(P.steps, w, kMaxSubWindow, subs, lps); *n_sub += (int)subs.size(); }
 }
+
+// The one-body window passes exactly as the library plans them (mask windows,
+// link tables, per-pass coefficients, split-window fiber signs), with plain
+// loops: out = sum_{pq} M_pq (E^alpha_pq + E^beta_pq) vec.  w: window width.
+// Returns the number of passes per sector (0: the library uses its gather path).
+extern "C" int emulate_one_body(const double* vec, double* out, int norb, int na, int nb,
+                                const double* mat, int w) {
+  auto passes = one_body_passes(norb, w);
+  if (passes.empty()) return 0;
+  const long da = binom(norb, na), db = binom(norb, nb);
+  const C* v = (const C*)vec; C* o = (C*)out; const C* M = (const C*)mat;
+  for (long k = 0; k < da * db; ++k) o[k] = 0.0;
+  for (int sector = 0; sector < 2; ++sector) {
+    const int nocc = sector == 0 ? na : nb;
+    const long n_batch = sector == 0 ? db : da;
+    for (auto& P : passes) {
+      auto L = make_mask_layout(norb, nocc, P.mask, P.between);
+      auto B = make_one_body_links(L.w);
+      std::vector<int> loc;
+      for (int b = 0; b < norb; ++b) if ((P.mask >> b) & 1) loc.push_back(b);
+      for (auto& fd : L.fibers) {
+        const int32_t* rows = L.row_table.data() + fd.row_off;
+        const int off = B.index[2 * fd.kp], nl = B.index[2 * fd.kp + 1];
+        for (int i = 0; i < fd.n_rows; ++i)
+          for (long b = 0; b < n_batch; ++b) {
+            C acc = 0.0;
+            for (int l = 0; l < nl; ++l) {
+              uint32_t u = B.links[off + (long)l * fd.n_rows + i];
+              int S = u & 0xffff, p = (u >> 16) & 31, q = (u >> 21) & 31;
+              int gp = loc[p], gq = loc[q];
+              if (!P.take[(size_t)gp * norb + gq]) continue;
+              C x = sector == 0 ? v[(long)rows[S] * db + b] : v[b * db + rows[S]];
+              acc += ((u >> 26) ? -1.0 : 1.0) * M[(size_t)gp * norb + gq] * x;
+            }
+            if (fd.pad) acc = -acc;
+            if (sector == 0) o[(long)rows[i] * db + b] += acc; else o[b * db + rows[i]] += acc;
+          }
+      }
+    }
+  }
+  return (int)passes.size();
+}

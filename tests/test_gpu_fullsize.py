@@ -277,3 +277,42 @@ def test_norb18_c64_trotter_step_error_vs_c128_and_round_trip():
     _check("c64", *_compare(t64, a, b, da, db))
     del t64, v64
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize(
+    "norb,nelec,dtype", [(16, (8, 8), "c128"), (16, (8, 8), "c64"), (14, (7, 6), "c128")]
+)
+def test_product_state_one_body_contraction(norb, nelec, dtype):
+    """contract_one_body with a random complex M on psi = a (x) b gives
+    (M a) (x) b + a (x) (M b), each factor from the oracle's single-sector
+    contraction (pyscf's contract_1e restated).  Above 12 orbitals the
+    contraction runs as three window passes per sector, one of them over a
+    split window."""
+    from ffsim_amd.contract.one_body import contract_one_body_into
+
+    torch = _torch()
+    rng = np.random.default_rng(norb * 10 + (dtype == "c64"))
+    a, b = _factors(norb, nelec, rng)
+    M = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    ma = O.contract_one_body(a, M, norb, (nelec[0], 0))
+    mb = O.contract_one_body(b, M, norb, (nelec[1], 0))
+    da, db = len(a), len(b)
+    _case("product_state_one_body", norb, nelec, dtype, da, db)
+    t = _product_state(a, b, dtype)
+    out = torch.empty_like(t)
+    contract_one_body_into(t, out, M, norb, nelec)
+    got = out.reshape(da, db)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    tma, tmb = torch.from_numpy(ma).cuda(), torch.from_numpy(mb).cuda()
+    max_err = max_ref = 0.0
+    for i0 in range(0, da, 1024):
+        i1 = min(da, i0 + 1024)
+        ref = tma[i0:i1, None] * tb[None, :] + ta[i0:i1, None] * tmb[None, :]
+        max_err = max(max_err, float((got[i0:i1].to(torch.complex128) - ref).abs().max()))
+        max_ref = max(max_ref, float(ref.abs().max()))
+    print(f"  max|delta|={max_err:.3e} max|ref|={max_ref:.3e}", flush=True)
+    if dtype == "c128":
+        assert max_err <= C128_TOL, max_err
+    else:
+        assert max_err <= C64_REL * max_ref, (max_err, max_ref)
+    del out, t

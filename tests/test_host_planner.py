@@ -40,6 +40,9 @@ def emulator(tmp_path_factory):
     lib.decompose.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]
     lib.decompose.restype = ctypes.c_int
     lib.plan_counts.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P]
+    lib.emulate_one_body.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                     ctypes.c_int]
+    lib.emulate_one_body.restype = ctypes.c_int
     return lib
 
 
@@ -122,3 +125,25 @@ def test_triangle_order_needs_fewer_passes(emulator, rng):
         assert got[0][0] <= got[1][0] and got[0][1] <= got[1][1], (n, got)
         if n in (17, 18):
             assert got[0][0] == 3 and got[1][0] == 4, (n, got)
+
+
+@pytest.mark.parametrize("norb,nelec,w,n_passes", [
+    (4, (2, 1), 4, 1), (6, (3, 2), 4, 3), (7, (3, 4), 5, 3), (8, (4, 4), 6, 3),
+    (9, (0, 5), 6, 3), (10, (5, 3), 8, 3), (8, (8, 2), 6, 3), (12, (6, 6), 12, 1)])
+def test_emulated_one_body_window_passes_match_oracle(emulator, norb, nelec, w, n_passes, rng):
+    """The one-body window passes (windows A+B, B+C and the split window A+C
+    whose fibers carry the sign of their electrons on B) against the oracle's
+    restatement of pyscf's contract_1e over both spin sectors."""
+    da, db = math.comb(norb, nelec[0]), math.comb(norb, nelec[1])
+    vec = O.random_state_vector(da * db, rng)
+    M = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    out = np.zeros_like(vec)
+    got = emulator.emulate_one_body(_p(vec), _p(out), norb, nelec[0], nelec[1], _p(M), w)
+    assert got == n_passes
+    np.testing.assert_allclose(out, O.contract_one_body(vec, M, norb, nelec), atol=1e-12, rtol=0)
+
+
+def test_one_body_passes_need_a_wide_enough_window(emulator):
+    vec = np.zeros(math.comb(10, 5) ** 2, dtype=complex)
+    M = np.eye(10, dtype=complex)
+    assert emulator.emulate_one_body(_p(vec), _p(vec.copy()), 10, 5, 5, _p(M), 6) == 0

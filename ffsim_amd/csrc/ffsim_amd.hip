@@ -1851,7 +1851,8 @@ int ffsim_diag_coulomb_grad_reduce(const void* psi, const void* lam, int dtype, 
     // column sums over row chunks (enough blocks to fill the device), then the
     // chunks of each column added in order
     const long col_blocks = (db + 255) / 256;
-    const long n_chunks = std::max(1l, std::min<long>(da, (4l * g_n_cu + col_blocks - 1) / col_blocks));
+    const long n_chunks =
+        std::max(1l, std::min<long>({da, 32l, (4l * g_n_cu + col_blocks - 1) / col_blocks}));
     const long rows_per_chunk = (da + n_chunks - 1) / n_chunks;
     Program prog(ctx, st);
     size_t poff = 0;

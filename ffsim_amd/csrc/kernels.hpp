@@ -1002,6 +1002,7 @@ static __global__ void __launch_bounds__(256)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cols) return;
   double s = 0.0;
+#pragma unroll 8
   for (int k = 0; k < n_chunks; ++k) s += part[(long)k * n_cols + c];
   csum[c] = s;
 }

@@ -147,3 +147,37 @@ def test_one_body_passes_need_a_wide_enough_window(emulator):
     vec = np.zeros(math.comb(10, 5) ** 2, dtype=complex)
     M = np.eye(10, dtype=complex)
     assert emulator.emulate_one_body(_p(vec), _p(vec.copy()), 10, 5, 5, _p(M), 6) == 0
+
+
+def test_emulated_passes_random_shapes(emulator):
+    """Random shapes, windows and rotation orders (hypothesis): the emulated
+    fused passes and one-body window passes against the oracle."""
+    hyp = pytest.importorskip("hypothesis")
+    st = hyp.strategies
+
+    @hyp.settings(max_examples=40, deadline=None, derandomize=True)
+    @hyp.given(norb=st.integers(2, 10), data=st.data())
+    def check(norb, data):
+        na = data.draw(st.integers(0, norb))
+        nb = data.draw(st.integers(0, norb))
+        w = data.draw(st.integers(2, norb))
+        clements = data.draw(st.integers(0, 1))
+        merge = data.draw(st.integers(0, 1))
+        seed = data.draw(st.integers(0, 2**31 - 1))
+        rng = np.random.default_rng(seed)
+        da, db = math.comb(norb, na), math.comb(norb, nb)
+        vec = O.random_state_vector(da * db, rng)
+        Ua, Ub = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+        x, work = vec.copy(), np.zeros_like(vec)
+        emulator.emulate_order(_p(x), _p(work), norb, na, nb, _p(Ua), _p(Ub), w, merge, clements)
+        np.testing.assert_allclose(x, O.apply_orbital_rotation(vec, (Ua, Ub), norb, (na, nb)),
+                                   atol=1e-12, rtol=0)
+        M = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+        out = np.zeros_like(vec)
+        n = emulator.emulate_one_body(_p(vec), _p(out), norb, na, nb, _p(M), w)
+        assert n == (1 if norb <= w else 3 if 2 * (norb - w) <= w else 0)
+        if n:
+            np.testing.assert_allclose(out, O.contract_one_body(vec, M, norb, (na, nb)),
+                                       atol=1e-12, rtol=0)
+
+    check()

@@ -751,7 +751,8 @@ def main() -> int:
                     help="skip the c64-vs-c128 comparison (keeps profiler runs to one dtype)")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--no-gradient", action="store_true")
-    ap.add_argument("--only", choices=["gates", "trotter", "gradient", "batched", "serving"],
+    ap.add_argument("--only",
+                    choices=["gates", "numpy", "trotter", "gradient", "batched", "serving"],
                     default=None,
                     help="after the headline run, run this one leg and no other (for profiling)")
     ap.add_argument("--no-extras", action="store_true",
@@ -897,6 +898,8 @@ def main() -> int:
             result["trotter_norb18"] = trotter_leg()
         elif args.only == "gradient":
             result["gradient_norb12"] = gradient_leg()
+        elif args.only == "numpy":
+            result["numpy_io"] = numpy_io_leg(op, norb, nelec)
         elif args.only == "batched":
             result["batched_lucj_paths"] = batched_paths_leg()
         else:

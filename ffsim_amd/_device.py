@@ -66,15 +66,33 @@ def host_ptr(a):
 _STAGE_MIN_BYTES = 1 << 20
 
 
+# Large host-to-device copies run in chunks through two pinned buffers, so
+# that the host copy of one chunk overlaps the transfer of the previous one.
+_CHUNK_BYTES = 64 << 20
+
+
 def host_to_device(arr: np.ndarray, dev: torch.device) -> torch.Tensor:
     """Copy a C-contiguous 1-D host array to a new device tensor (current stream)."""
     src = torch.from_numpy(arr)
     if arr.nbytes < _STAGE_MIN_BYTES:
         return src.to(dev)
-    staged = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
-    staged.copy_(src)
-    # the caching host allocator keeps `staged` until the copy has run
-    return staged.to(dev, non_blocking=True)
+    out = torch.empty(src.shape, dtype=src.dtype, device=dev)
+    n = src.numel()
+    per = max(1, _CHUNK_BYTES // src.element_size())
+    bufs = [torch.empty(min(per, n), dtype=src.dtype, pin_memory=True) for _ in range(2)]
+    done = [None, None]
+    stream = torch.cuda.current_stream(dev)
+    for k, i0 in enumerate(range(0, n, per)):
+        i1 = min(n, i0 + per)
+        slot = k % 2
+        if done[slot] is not None:
+            done[slot].synchronize()  # the buffer's previous transfer has run
+        bufs[slot][: i1 - i0].copy_(src[i0:i1])
+        out[i0:i1].copy_(bufs[slot][: i1 - i0], non_blocking=True)
+        done[slot] = torch.cuda.Event()
+        done[slot].record(stream)
+    # the caching host allocator keeps the pinned buffers until their copies have run
+    return out
 
 
 def start_device_to_host(t: torch.Tensor) -> torch.Tensor:
@@ -94,12 +112,14 @@ def finish_device_to_host(staged: torch.Tensor) -> np.ndarray:
 
 
 def device_to_host(t: torch.Tensor) -> np.ndarray:
-    """Synchronous copy of a device tensor into a new numpy array."""
+    """Synchronous copy of a device tensor into a new numpy array.  A large state
+    lands in pinned host memory that the returned array keeps (no second host
+    copy); the memory goes back to torch's pinned cache when the array is freed."""
     if t.numel() * t.element_size() < _STAGE_MIN_BYTES:
         return t.cpu().numpy()
     staged = start_device_to_host(t)
     torch.cuda.current_stream().synchronize()
-    return finish_device_to_host(staged)
+    return staged.numpy()
 
 
 @dataclass

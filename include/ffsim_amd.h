@@ -162,7 +162,9 @@ int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void*
 /* One-body contraction: out (+)= scale * sum_{pq, sigma} mat[p][q] a+_{p sigma} a_{q sigma} vec,
  * mat a HOST norb x norb complex128 matrix.  Replaces contract_one_body
  * (python/ffsim/contract/one_body.py:24-56, i.e. pyscf's contract_1e over the link
- * tables of _cistring.py:33-42).  out must not alias vec.  norb <= 64. */
+ * tables of _cistring.py:33-42).  Up to norb = 18 it runs as window passes
+ * (one per sector up to 12 orbitals, three above); wider states use a gather
+ * over link tables.  out must not alias vec.  norb <= 63. */
 int ffsim_contract_one_body(const void* vec, void* out, int dtype, int norb, int n_alpha,
                             int n_beta, const double* mat, double scale, int accumulate,
                             void* stream);
@@ -179,7 +181,7 @@ int ffsim_diag_coulomb_grad_reduce(const void* psi, const void* lam, int dtype, 
 
 /* Transition one-body RDM rdm_sigma[p][q] = <lam| a+_{p sigma} a_{q sigma} |psi> for
  * sigma = alpha then beta, as 2 x norb x norb complex128 in the DEVICE buffer out
- * (zeroed by the call). */
+ * (overwritten).  Sums run in a fixed order: repeated calls give the same bits. */
 int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, int n_alpha,
                          int n_beta, double* out, void* stream);
 

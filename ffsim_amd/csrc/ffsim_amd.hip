@@ -724,21 +724,20 @@ void launch_fiber(int mode, const void* src, void* dst, long ld_src, long ld_dst
                                        fz ? fz->n_fac : 0, fz ? fz->win : 0);
   const bool wide = 2 * lds_l > lds_max;
   if (fz) {
-    // the fused diagonal store exists for the c128 production shape only
-    if constexpr (std::is_same<T, double>::value) {
-      require(W == 8 && mode == MODE_ROWS && gr.w2 == kMaxSubWindow && tstore, FFSIM_ERR_VALUE,
-              "fused diagonal: unsupported pass");
-      if (wide)
-        launch_fiber_t<double, 8, MODE_ROWS, kMaxSubWindow, 512, true>(
-            src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
-            d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
-      else
-        launch_fiber_t<double, 8, MODE_ROWS, kMaxSubWindow, 256, true>(
-            src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
-            d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
-      return;
-    }
-    throw Failure{FFSIM_ERR_VALUE, "fused diagonal: complex128 only"};
+    // the fused diagonal store exists for the production tile widths (8
+    // columns for complex128, 4 for complex64) of a transposing row pass
+    constexpr int WF = std::is_same<T, double>::value ? 8 : 4;
+    require(W == WF && mode == MODE_ROWS && gr.w2 == kMaxSubWindow && tstore, FFSIM_ERR_VALUE,
+            "fused diagonal: unsupported pass");
+    if (wide)
+      launch_fiber_t<T, WF, MODE_ROWS, kMaxSubWindow, 512, true>(
+          src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
+          d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
+    else
+      launch_fiber_t<T, WF, MODE_ROWS, kMaxSubWindow, 256, true>(
+          src, dst, ld_src, ld_dst, n_batch, tstore, win, all_fibers, gr, los, gd_cap, gm_cap,
+          d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream, *fz);
+    return;
   }
 #define FFSIM_FIBER_LAUNCH(WW, MM, W2)               \
   do {                                               \
@@ -972,13 +971,14 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     // fuse the diagonal into this pass when the kernel's fused layout applies
     bool fz_on = false;
     // (the fused store takes the pass's window to be [0, w), w <= 16)
-    if (ts && fuse && dtype == FFSIM_C128 && mode == MODE_ROWS && tile_width(dtype) == 8 &&
-        gr.w2 == kMaxSubWindow && n_sub > 0 && P.lo == 0 && w <= 16) {
+    const int wf = dtype == FFSIM_C128 ? 8 : 4;
+    if (ts && fuse && mode == MODE_ROWS && tile_width(dtype) == wf && gr.w2 == kMaxSubWindow &&
+        n_sub > 0 && P.lo == 0 && w <= 16) {
       const int rows = win.max_rows;
-      const size_t lds = fiber_lds_bytes(16, 8, false, 16, rows, gd_cap, gm_cap, dg, true, norb,
-                                         fuse->n_fac, w);
+      const size_t lds = fiber_lds_bytes(elem_bytes(dtype), wf, false, 16, rows, gd_cap, gm_cap,
+                                         dg, true, norb, fuse->n_fac, w);
       const int nthr = 2 * lds > ctx.lds_max ? 512 : 256;
-      fz_on = lds <= ctx.lds_max && rows <= 2 * nthr && 8 * fuse->n_fac <= nthr;
+      fz_on = lds <= ctx.lds_max && rows <= 1024 && wf * fuse->n_fac <= nthr;
     }
     if (fz_on && fused) *fused = true;
     const FuseTables ft = fz_on ? *fuse : FuseTables{};
@@ -1025,7 +1025,7 @@ bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
   // batch columns, beta strings its tile rows
   FuseTables ft;
   bool want_fuse = then_diag && then_diag->flags && !then_diag->z_rep &&
-                   then_diag->norb == norb && dtype == FFSIM_C128 && beta_transposed &&
+                   then_diag->norb == norb && beta_transposed &&
                    norb + 1 <= 64;
   if (want_fuse) {
     ft.n_fac = norb + 1;

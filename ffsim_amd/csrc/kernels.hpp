@@ -370,13 +370,17 @@ __global__ void __launch_bounds__(NTHR)
   // The group tables and wave schedule depend only on the fiber's electron count
   // in the window (kp); `tables` is false when the previous fiber had the same kp.
   // fused diagonal: the tile strings' factors of this thread's store rows
-  // (i = tid, tid + NTHR), fixed while the fiber is
-  Cx<double> rf[2] = {{1.0, 0.0}, {1.0, 0.0}};
+  // (i = tid + k NTHR; fibers of a fused pass hold at most 1024 strings), fixed
+  // while the fiber is
+  constexpr int RF = FUSE ? (1024 + NTHR - 1) / NTHR : 1;
+  Cx<double> rf[RF];
+#pragma unroll
+  for (int k = 0; k < RF; ++k) rf[k] = Cx<double>{1.0, 0.0};
   auto stage_fiber = [&](const Fiber& fd, bool tables) {
     for (int i = tid; i < fd.n_rows; i += NTHR) rows[i] = row_table[fd.row_off + i];
     if (FUSE) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < RF; ++k) {
         const int i = tid + k * NTHR;
         rf[k] = i < fd.n_rows ? fz.rfac[row_table[fd.row_off + i]] : Cx<double>{1.0, 0.0};
       }
@@ -548,8 +552,11 @@ __global__ void __launch_bounds__(NTHR)
         ntab[k * W + j] = P;
       }
       __syncthreads();
-      for (int i = tid; i < nr; i += NTHR) {
-        const Cx<double> r0f = i < NTHR ? rf[0] : rf[1];
+#pragma unroll
+      for (int k = 0; k < RF; ++k) {
+        const int i = tid + k * NTHR;
+        if (i >= nr) break;
+        const Cx<double> r0f = rf[k];
         const uint32_t sv = (uint32_t)(narrow_str ? (uint64_t)rstr32[i] : rstr64[i]);
         Cx<double> P[W];
 #pragma unroll

@@ -195,8 +195,8 @@ int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, 
  * for times[t]; finally rotate by rotations[n_terms].
  * rotations: (n_terms + 1) x norb x norb complex128; mats: n_terms x 3 x norb x
  * norb real, prepared as for ffsim_apply_diag_coulomb_evolution (number
- * representation: aa/bb diagonals halved); times: n_terms.  In complex128 and
- * the number representation, each evolution is applied inside the last pass
+ * representation: aa/bb diagonals halved); times: n_terms.  In the number
+ * representation, each evolution is applied inside the last pass
  * of the rotation before it when the beta sector runs transposed (results are
  * the same to rounding; no separate diagonal pass over the state). */
 int ffsim_apply_diag_coulomb_sequence(void* vec, void* work, int dtype, int norb, int n_alpha,

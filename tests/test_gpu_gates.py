@@ -69,6 +69,23 @@ def test_orbital_rotation_matches_oracle(norb, nelec, rng):
         np.testing.assert_allclose(got, expected, atol=TOL, rtol=0)
 
 
+@pytest.mark.parametrize("order", ["clements", "triangle"])
+@pytest.mark.parametrize("norb,nelec", [(8, (4, 3)), (13, (6, 5)), (14, (7, 7)), (15, (5, 6))])
+def test_orbital_rotation_in_either_rotation_order(norb, nelec, order, rng, monkeypatch):
+    """FFSIM_AMD_GIVENS_ORDER forces the rotation order of the fused passes
+    (the library picks the one with fewer passes, then layer pairs): both give
+    the oracle's result."""
+    import ffsim_amd
+
+    monkeypatch.setenv("FFSIM_AMD_GIVENS_ORDER", order)
+    _case("orbital_rotation order", norb, nelec, order)
+    vec = _rand(norb, nelec, rng)
+    Ua, Ub = O.random_unitary(norb, rng), O.random_unitary(norb, rng)
+    expected = O.apply_orbital_rotation(vec, (Ua, Ub), norb, nelec)
+    got = ffsim_amd.apply_orbital_rotation(vec, (Ua, Ub), norb, nelec)
+    np.testing.assert_allclose(got, expected, atol=TOL, rtol=0)
+
+
 @pytest.mark.parametrize("norb,nelec", [(0, (0, 0)), (1, (0, 0)), (1, (1, 0)), (1, (0, 1)),
                                         (1, (1, 1)), (2, (1, 1)), (2, (2, 2)), (3, (2, 1)),
                                         (3, (1, 2)), (4, (3, 2))])

@@ -208,6 +208,9 @@ struct Context {
   // one-body window passes: fibers of (norb, nocc, window mask, between mask)
   // and the link tables of each window width
   std::map<std::tuple<int, int, uint64_t, uint64_t>, DevWindow> mask_windows;
+  // streams on which the Givens path of a batch runs several small states at
+  // once (created on first use, default priority)
+  std::vector<hipStream_t> aux;
   std::map<int, std::pair<uint32_t*, int32_t*>> ob_links;
   ProgramRing ring;
 };
@@ -1420,6 +1423,8 @@ void ucj_batched_gemm(Context& ctx, int n_states, void* const* vecs, void* const
   }
 }
 
+constexpr long kMaxAuxStreams = 4;
+
 void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const* works,
                       int dtype, int n, int na, int nb, int n_reps,
                       const double* diag_coulomb_mats, const double* orbital_rotations,
@@ -1438,11 +1443,53 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
                               orbital_rotations, final_rots, stream);
     return;
   }
+  // A launch over a small state has fewer tiles than the device has CUs, so
+  // several states run at once, one per auxiliary stream, between events on
+  // the caller's stream (each state's work stays in order on its stream).
+  // Only for calls on the default stream: a caller that runs its own streams
+  // (the server) already keeps the device's few hardware queues busy, and
+  // more streams than queues only add false dependencies (measured: serving
+  // saturation 1.17x instead of 1.25x with them).
+  const long tiles = std::max(da, db) / 8;
+  const int n_aux = (n_states < 2 || stream != nullptr) ? 0
+                                 : (int)std::min<long>({kMaxAuxStreams, n_states,
+                                                        std::max<long>(1, g_n_cu / std::max(1l, tiles))});
+  if (n_aux < 2) {
+    for (int i = 0; i < n_states; ++i)
+      ucj_impl(ctx, vecs[i], works[i], dtype, n, na, nb, n_reps,
+               diag_coulomb_mats + (size_t)i * n_reps * 2 * nn,
+               orbital_rotations + (size_t)i * n_reps * 2 * nn,
+               final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, stream);
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    while ((int)ctx.aux.size() < n_aux) {
+      hipStream_t s = nullptr;
+      HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      ctx.aux.push_back(s);
+    }
+  }
+  auto event = [] {
+    hipEvent_t e = nullptr;
+    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+  };
+  hipEvent_t start = event();
+  HIP_CHECK(hipEventRecord(start, stream));
+  for (int k = 0; k < n_aux; ++k) HIP_CHECK(hipStreamWaitEvent(ctx.aux[k], start, 0));
   for (int i = 0; i < n_states; ++i)
     ucj_impl(ctx, vecs[i], works[i], dtype, n, na, nb, n_reps,
              diag_coulomb_mats + (size_t)i * n_reps * 2 * nn,
              orbital_rotations + (size_t)i * n_reps * 2 * nn,
-             final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, stream);
+             final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, ctx.aux[i % n_aux]);
+  for (int k = 0; k < n_aux; ++k) {
+    hipEvent_t done = event();
+    HIP_CHECK(hipEventRecord(done, ctx.aux[k]));
+    HIP_CHECK(hipStreamWaitEvent(stream, done, 0));
+    HIP_CHECK(hipEventDestroy(done));  // released once it has completed
+  }
+  HIP_CHECK(hipEventDestroy(start));
 }
 }  // namespace
 }  // namespace ffsim_amd

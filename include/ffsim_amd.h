@@ -152,7 +152,9 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
  * batches of 8 or more) take a GEMM path: each merged orbital rotation is the
  * compound matrix Lambda^k(U) (built on the device) applied to all states by
  * batched complex GEMMs, and each diagonal term one batched elementwise pass;
- * other states run the Givens path state by state on the stream. */
+ * other states run the Givens path state by state on the stream (on the
+ * default stream, small states run on up to 4 library-owned streams at once,
+ * between events on the default stream). */
 int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void* const* works,
                                           int dtype, int norb, int n_alpha, int n_beta,
                                           int n_reps, const double* diag_coulomb_mats,

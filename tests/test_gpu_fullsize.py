@@ -135,7 +135,8 @@ def _phase_block(mats, time, norb, nelec):
 
 @pytest.mark.parametrize(
     "norb,nelec,dtype",
-    [(16, (8, 8), "c128"), (16, (8, 8), "c64"), (18, (9, 9), "c128"), (18, (9, 9), "c64")],
+    [(16, (8, 8), "c128"), (16, (8, 8), "c64"), (18, (9, 9), "c128"), (18, (9, 9), "c64"),
+     (17, (8, 9), "c128")],
 )
 def test_product_state_rotation(norb, nelec, dtype):
     import ffsim_amd

@@ -564,7 +564,8 @@ def cpu_baseline_gradient(threads: int, seed: int = 12) -> dict:
 
 def batched_paths_leg(reps: int = 5, seed: int = 31) -> dict:
     """The batched LUCJ entry (n_reps=2) on the compound-matrix GEMM path against
-    the Givens path, per shape and batch size, complex128 states resident in HBM:
+    the Givens path with all states in each launch ("sweep") and state by state
+    ("states"), per shape and batch size, complex128 states resident in HBM:
     milliseconds per state (wall time of the call and a synchronize, median of
     ``reps`` after a warm-up)."""
     import torch
@@ -583,7 +584,7 @@ def batched_paths_leg(reps: int = 5, seed: int = 31) -> dict:
             states = [torch.randn(dim, dtype=torch.complex128, device="cuda") for _ in range(batch)]
             works = [torch.empty_like(t) for t in states]
             row = {}
-            for path in ("gemm", "sweep"):
+            for path in ("gemm", "sweep", "states"):
                 os.environ["FFSIM_AMD_BATCH_PATH"] = path
                 times = []
                 for k in range(reps + 1):

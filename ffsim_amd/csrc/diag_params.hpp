@@ -71,12 +71,15 @@ inline std::vector<double> diag_coefficients(const DiagSpec& spec) {
 constexpr int kDiagRows = 8;
 constexpr int kDiagColsPerBlock = 2048;
 
-// The column-term pre-pass: gamma[c] (contractions) or its phase factor (evolution).
+// The column-term pre-pass: gamma[c] (contractions) or its phase factor
+// (evolution), for n_ops operators at the given strides (headers consecutive).
 inline void launch_diag_col_term(long n_cols, const uint64_t* col_str, const DiagHeader* hdr,
                                  const double* coef, double* gamma, int evolve,
-                                 hipStream_t stream) {
-  hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((n_cols + 255) / 256)), dim3(256), 0,
-                     stream, (int)n_cols, col_str, hdr, coef, gamma, evolve);
+                                 hipStream_t stream, int n_ops = 1, long coef_stride = 0,
+                                 long gamma_stride = 0) {
+  hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((n_cols + 255) / 256), (unsigned)n_ops),
+                     dim3(256), 0, stream, (int)n_cols, col_str, hdr, coef, gamma, evolve,
+                     coef_stride, gamma_stride);
 }
 
 // The elementwise pass over an n_rows x n_cols state (blocks of kDiagRows rows x
@@ -88,16 +91,23 @@ inline dim3 diag_grid(long n_rows, long n_cols) {
               (unsigned)((n_cols + cols_blk - 1) / cols_blk));
 }
 
+// With `states`, n_states states are transformed in place (src and dst
+// unused), state z by the operator at hdr + z, coef + z coef_stride and
+// gamma + z gamma_stride.
 template <typename T>
 void launch_diag_pass_t(int op, const void* src, void* dst, long n_rows, long n_cols,
                         const uint64_t* row_str, const uint64_t* col_str, const double* gamma,
-                        const DiagHeader* hdr, const double* coef, hipStream_t stream) {
+                        const DiagHeader* hdr, const double* coef, hipStream_t stream,
+                        void* const* states = nullptr, int n_states = 1, long coef_stride = 0,
+                        long gamma_stride = 0) {
   const int cols_blk = (int)std::min<long>(n_cols, kDiagColsPerBlock);
-  const dim3 grid = diag_grid(n_rows, n_cols);
+  dim3 grid = diag_grid(n_rows, n_cols);
+  grid.z = (unsigned)n_states;
 #define FFSIM_DIAG_LAUNCH(MM)                                                                \
   hipLaunchKernelGGL((diag_kernel<T, kDiagRows, MM>), grid, dim3(256), 0, stream,            \
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), n_cols, \
-                     (int)n_rows, (int)n_cols, cols_blk, row_str, col_str, gamma, hdr, coef)
+                     (int)n_rows, (int)n_cols, cols_blk, row_str, col_str, gamma, hdr, coef,    \
+                     reinterpret_cast<Cx<T>* const*>(states), coef_stride, gamma_stride)
   if (op == DIAG_EVOLVE)
     FFSIM_DIAG_LAUNCH(DIAG_EVOLVE);
   else if (op == DIAG_CONTRACT)

@@ -54,9 +54,12 @@ __global__ void fill_zero_kernel(double* __restrict__ x, long n) {
 __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ col_str,
                                      const DiagHeader* __restrict__ hdr,
                                      const double* __restrict__ coef, double* __restrict__ gamma,
-                                     int evolve) {
+                                     int evolve, long coef_stride, long gamma_stride) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cols) return;
+  hdr += blockIdx.y;
+  coef += blockIdx.y * coef_stride;
+  gamma += blockIdx.y * gamma_stride;
   const int n = hdr->norb;
   const int z = hdr->z_rep;
   const int flags = hdr->flags;
@@ -703,9 +706,75 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
                      ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(sched), los,
                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap,
-                     gm_cap, fz);
+                     gm_cap, fz, MultiTiles{nullptr, 1, 0, 0});
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
+}
+
+// One window pass over several states of one shape (in place, no transposing
+// store, no fused diagonal): the states' tiles in one persistent launch.
+template <typename T, int W, int MODE, int NTHR>
+void launch_fiber_multi_t(const MultiTiles& mt, long ld, int n_batch, const DevWindow& win,
+                          bool all_fibers, const DevGroups& gr, const LoSlots& los, int gd_cap,
+                          int gm_cap, const SubPass* d_subs, int n_sub, const LayerPair* d_layers,
+                          int n_layers, const double* d_diag, const uint64_t* d_strings,
+                          int n_diag, size_t lds_max, hipStream_t stream) {
+  const int n_f = all_fibers ? win.n_fibers : win.n_active;
+  if (n_f == 0 || n_batch == 0 || mt.n_states == 0) return;
+  const int rows = all_fibers ? win.max_rows : win.max_rows_active;
+  const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, MODE == MODE_COLS, NTHR / 64, rows,
+                                     gd_cap, gm_cap, d_diag != nullptr, false, n_diag, 0, 0);
+  require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
+  auto kern = fiber_kernel<T, W, MODE, NTHR, kMaxSubWindow, false, true>;
+  const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_max));
+    attr_set = true;
+  }
+  const int n_bt = (n_batch + W - 1) / W;
+  const long n_tiles = (long)n_bt * n_f * mt.n_states;
+  require(n_tiles < (1l << 40), FFSIM_ERR_VALUE, "grid too large");
+  int occ = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, NTHR, lds));
+  const long grid = std::min<long>(n_tiles, (long)std::max(1, occ) * g_n_cu);
+  const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch * (double)mt.n_states *
+                       (double)(all_fibers ? win.rows_all : win.rows_active);
+  const long pk = prof_start(PROF_FIBER, moved, stream);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
+                     static_cast<const Cx<T>*>(nullptr), static_cast<Cx<T>*>(nullptr), ld, ld,
+                     n_batch, 0, win.fibers, n_f, win.rows, gr.args(sched), los, d_subs, n_sub,
+                     d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap, gm_cap,
+                     DiagFuse{}, mt);
+  HIP_CHECK(hipGetLastError());
+  prof_stop(pk, stream);
+}
+
+template <typename T>
+void launch_fiber_multi(int mode, const MultiTiles& mt, long ld, int n_batch,
+                        const DevWindow& win, bool all_fibers, const DevGroups& gr,
+                        const LoSlots& los, int gd_cap, int gm_cap, const SubPass* d_subs,
+                        int n_sub, const LayerPair* d_layers, int n_layers, const double* d_diag,
+                        const uint64_t* d_strings, int n_diag, size_t lds_max, int W,
+                        hipStream_t stream) {
+  require(gr.w2 == kMaxSubWindow && W == (std::is_same<T, double>::value ? 8 : 4),
+          FFSIM_ERR_VALUE, "multi-state pass: unsupported shape");
+  constexpr int WW = std::is_same<T, double>::value ? 8 : 4;
+  const int rows_l = all_fibers ? win.max_rows : win.max_rows_active;
+  const size_t lds_l = fiber_lds_bytes(sizeof(Cx<T>), WW, mode == MODE_COLS, 16, rows_l, gd_cap,
+                                       gm_cap, d_diag != nullptr, false, n_diag, 0, 0);
+  const bool wide = 2 * lds_l > lds_max;
+#define FFSIM_MULTI_LAUNCH(MM, NT)                                                           \
+  launch_fiber_multi_t<T, WW, MM, NT>(mt, ld, n_batch, win, all_fibers, gr, los, gd_cap, gm_cap, \
+                                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings,   \
+                                      n_diag, lds_max, stream)
+  if (mode == MODE_ROWS) {
+    if (wide) FFSIM_MULTI_LAUNCH(MODE_ROWS, 512); else FFSIM_MULTI_LAUNCH(MODE_ROWS, 256);
+  } else {
+    if (wide) FFSIM_MULTI_LAUNCH(MODE_COLS, 512); else FFSIM_MULTI_LAUNCH(MODE_COLS, 256);
+  }
+#undef FFSIM_MULTI_LAUNCH
 }
 
 template <typename T>
@@ -916,6 +985,30 @@ struct FuseTables {
   int n_fac = 0;
 };
 
+// The sub-window slots of a pass and the largest group tables that any fiber
+// of the pass stages (over the window's electron counts kp).
+void pass_slots(const DevGroups& gr, const std::vector<SubPass>& subs, LoSlots& los, int& gd_cap,
+                int& gm_cap) {
+  los = LoSlots{};
+  for (const SubPass& sp : subs) {
+    require(sp.slot < kMaxLoSlots, FFSIM_ERR_VALUE, "too many sub-window starts");
+    los.lo2[sp.slot] = sp.lo2;
+    los.n = std::max(los.n, sp.slot + 1);
+  }
+  gd_cap = 0;
+  gm_cap = 0;
+  for (int kp = 0; kp <= gr.w; ++kp) {
+    int nd = 0, nm = 0;
+    for (int u = 0; u < los.n; ++u) {
+      const int* ix = gr.host_index.data() + ((size_t)kp * gr.n_lo2 + los.lo2[u]) * 4;
+      nd += ix[1];
+      nm += ix[3];
+    }
+    gd_cap = std::max(gd_cap, nd);
+    gm_cap = std::max(gm_cap, nm);
+  }
+}
+
 void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, long ld,
                 int n_batch, int norb, int nocc, const SectorRotation& rot, void* tstore_dst,
                 long ld_t, hipStream_t stream, std::vector<std::function<void()>>& launches,
@@ -951,22 +1044,8 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
     const int n_layers = (int)sub_layers.size();
     const int n_sub = (int)subs.size();
     LoSlots los{};
-    for (const SubPass& sp : subs) {
-      require(sp.slot < kMaxLoSlots, FFSIM_ERR_VALUE, "too many sub-window starts");
-      los.lo2[sp.slot] = sp.lo2;
-      los.n = std::max(los.n, sp.slot + 1);
-    }
     int gd_cap = 0, gm_cap = 0;
-    for (int kp = 0; kp <= gr.w; ++kp) {
-      int nd = 0, nm = 0;
-      for (int u = 0; u < los.n; ++u) {
-        const int* ix = gr.host_index.data() + ((size_t)kp * gr.n_lo2 + los.lo2[u]) * 4;
-        nd += ix[1];
-        nm += ix[3];
-      }
-      gd_cap = std::max(gd_cap, nd);
-      gm_cap = std::max(gm_cap, nm);
-    }
+    pass_slots(gr, subs, los, gd_cap, gm_cap);
     const DevWindow& win = dev_window(ctx, norb, nocc, P.lo, std::max(w, 1));
     void* dst = ts ? tstore_dst : state;
     const long ld_dst = ts ? ld_t : ld;
@@ -1049,7 +1128,7 @@ bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
       HIP_CHECK(hipGetLastError());
       hipLaunchKernelGGL(diag_col_term_kernel, dim3((unsigned)((dim_b + 255) / 256)), dim3(256),
                          0, stream, (int)dim_b, sb.d, prog.dev<DiagHeader>(hoff),
-                         prog.dev<double>(coff), prog.scratch<double>(ft.rfac_off), 1);
+                         prog.dev<double>(coff), prog.scratch<double>(ft.rfac_off), 1, 0l, 0l);
       HIP_CHECK(hipGetLastError());
     });
   }
@@ -1243,22 +1322,27 @@ void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int
 // Where the GEMM path is taken by default.  An orbital rotation there costs
 // 8 dim^3 flops per state and sector against one HBM pass of the state on the
 // Givens path, and the path's fixed cost (a compound-matrix launch and two
-// batched GEMMs per rotation) is shared by the batch.  Measured per state
-// (LUCJ n_reps=2, complex128, MI355X): norb=8 (70 strings) 0.15 vs 0.24 ms
-// alone and 0.014 vs 0.21 ms in a batch of 32; norb=10 (252 strings) 0.62 vs
-// 0.31 ms alone but 0.078 vs 0.25 ms in a batch of 32; norb=12 (924 strings)
-// 2.6 vs 0.44 ms even in a batch of 32.
+// batched GEMMs per rotation) is shared by the batch, as the Givens path's
+// launches are when all states share them.  Measured per state (LUCJ
+// n_reps=2, complex128, MI355X; GEMM vs Givens with all states per launch):
+// norb=8 (70 strings) 0.29 vs 0.57 ms alone and 0.014 vs 0.031 ms in a batch
+// of 32; norb=10 (252 strings) 0.37 vs 0.32 ms alone and 0.073 vs 0.061 ms in
+// a batch of 32; norb=12 (924 strings) 2.6 vs 0.20 ms in a batch of 32.  The
+// dim^3 against dim^2 scaling puts the crossover in batches near 150 strings.
 constexpr long kGemmAlwaysDim = 100;   // any batch
-constexpr long kGemmBatchDim = 256;    // batches of at least kGemmMinBatch states
+constexpr long kGemmBatchDim = 150;    // batches of at least kGemmMinBatch states
 constexpr int kGemmMinBatch = 8;
 
-// FFSIM_AMD_BATCH_PATH=gemm | sweep forces a path (read at every call, for
-// measurements that compare the two).
+// FFSIM_AMD_BATCH_PATH=gemm | sweep | states forces a path (read at every
+// call, for measurements that compare them): the compound-matrix GEMMs, the
+// Givens passes over all states at once (where they apply), or the Givens
+// passes state by state.
 int batch_path_override() {
   const char* e = std::getenv("FFSIM_AMD_BATCH_PATH");
   if (!e) return 0;
   if (std::string(e) == "gemm") return 1;
   if (std::string(e) == "sweep") return 2;
+  if (std::string(e) == "states") return 3;
   return 0;
 }
 
@@ -1266,7 +1350,7 @@ bool gemm_path(int norb, int na, int nb, int n_states) {
   const long da = (long)binom(norb, na), db = (long)binom(norb, nb);
   const bool ok = na <= kMaxCompoundK && nb <= kMaxCompoundK && da <= 1024 && db <= 1024;
   if (batch_path_override() == 1) return ok;
-  if (batch_path_override() == 2) return false;
+  if (batch_path_override() >= 2) return false;
   const long d = std::max(da, db);
   return ok && (d <= kGemmAlwaysDim || (d <= kGemmBatchDim && n_states >= kGemmMinBatch));
 }
@@ -1423,6 +1507,232 @@ void ucj_batched_gemm(Context& ctx, int n_states, void* const* vecs, void* const
   }
 }
 
+// One sector rotation of every state of a batch, planned for multi-state
+// window passes: the pass structure (window starts, sub-passes, layer-pair
+// counts) is common to the states, and each state's layer pairs and orbital
+// phases follow at fixed strides.
+struct MultiSector {
+  struct Pass {
+    int lo = 0;
+    bool dg = false;  // applies the orbital phases (the last pass)
+    std::vector<SubPass> subs;
+    int n_layers = 0;
+    std::vector<LayerPair> layers;  // state-major, n_layers per state
+  };
+  std::vector<Pass> passes;
+  bool need_diag = false;
+  std::vector<double> diag;  // state-major, (re, im) of norb phases per state
+};
+
+// Plans the passes of the states' rotations (in place, window width w);
+// false when the states' pass structures differ (a rotation with vanishing
+// Givens angles skips rotations that the others apply).
+bool plan_multi_sector(const std::vector<const SectorRotation*>& rots, int norb, int w,
+                       MultiSector& ms) {
+  const size_t m = rots.size();
+  std::vector<std::vector<PlannedPass>> all(m);
+  for (size_t i = 0; i < m; ++i) {
+    all[i] = plan_passes(rots[i]->steps, norb, w, false, -1);
+    if (all[i].size() != all[0].size()) return false;
+    if (!rots[i]->trivial_diag) ms.need_diag = true;
+  }
+  if (ms.need_diag) {
+    ms.diag.assign(m * 2 * norb, 0.0);
+    for (size_t i = 0; i < m; ++i)
+      for (int k = 0; k < norb; ++k) {
+        ms.diag[(i * norb + k) * 2] = rots[i]->diag[k].real();
+        ms.diag[(i * norb + k) * 2 + 1] = rots[i]->diag[k].imag();
+      }
+  }
+  std::vector<SubPass> subs;
+  std::vector<LayerPair> layers;
+  for (size_t k = 0; k < all[0].size(); ++k) {
+    const PlannedPass& P0 = all[0][k];
+    for (size_t i = 1; i < m; ++i) {
+      const PlannedPass& Pi = all[i][k];
+      if (Pi.lo != P0.lo || Pi.last != P0.last || Pi.steps.empty() != P0.steps.empty())
+        return false;
+    }
+    MultiSector::Pass out;
+    out.lo = P0.lo;
+    out.dg = P0.last && ms.need_diag;
+    if (P0.steps.empty() && !out.dg) continue;
+    for (size_t i = 0; i < m; ++i) {
+      plan_sub_passes(all[i][k].steps, w, kMaxSubWindow, subs, layers);
+      if (i == 0) {
+        out.subs = subs;
+        out.n_layers = (int)layers.size();
+        out.layers.reserve(m * layers.size());
+      } else if (subs.size() != out.subs.size() || (int)layers.size() != out.n_layers ||
+                 std::memcmp(subs.data(), out.subs.data(), subs.size() * sizeof(SubPass)) != 0) {
+        return false;
+      }
+      out.layers.insert(out.layers.end(), layers.begin(), layers.end());
+    }
+    ms.passes.push_back(std::move(out));
+  }
+  return true;
+}
+
+// Queues the passes of a planned multi-state sector rotation: one fiber
+// launch per window pass over the tiles of every state (`states`, a device
+// array of the m state addresses in the call's program).
+void queue_multi_sector(Context& ctx, Program& prog, const MultiSector& ms, int dtype, int mode,
+                        size_t states_off, int m, long ld, int n_batch, int norb, int nocc, int w,
+                        hipStream_t stream, std::vector<std::function<void()>>& launches) {
+  const DevStrings& ds = dev_strings(ctx, norb, nocc);
+  const DevGroups& gr = dev_groups(ctx, w, dtype == FFSIM_C64);
+  size_t diag_off = 0;
+  if (ms.need_diag) diag_off = prog.append(ms.diag.data(), ms.diag.size() * sizeof(double));
+  for (const MultiSector::Pass& P : ms.passes) {
+    const size_t soff = prog.append(P.layers.data(), P.layers.size() * sizeof(LayerPair));
+    const size_t poff = prog.append(P.subs.data(), P.subs.size() * sizeof(SubPass));
+    LoSlots los{};
+    int gd_cap = 0, gm_cap = 0;
+    pass_slots(gr, P.subs, los, gd_cap, gm_cap);
+    const DevWindow& win = dev_window(ctx, norb, nocc, P.lo, w);
+    const int n_sub = (int)P.subs.size(), n_layers = P.n_layers;
+    const bool dg = P.dg;
+    launches.push_back([=, &ctx, &prog, &win, &gr, &ds]() {
+      const MultiTiles mt{prog.dev<void*>(states_off), m, (long)n_layers, 2l * norb};
+      const double* d_diag = dg ? prog.dev<double>(diag_off) : nullptr;
+      if (dtype == FFSIM_C128)
+        launch_fiber_multi<double>(mode, mt, ld, n_batch, win, dg, gr, los, gd_cap, gm_cap,
+                                   prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff),
+                                   n_layers, d_diag, ds.d, norb, ctx.lds_max, tile_width(dtype),
+                                   stream);
+      else
+        launch_fiber_multi<float>(mode, mt, ld, n_batch, win, dg, gr, los, gd_cap, gm_cap,
+                                  prog.dev<SubPass>(poff), n_sub, prog.dev<LayerPair>(soff),
+                                  n_layers, d_diag, ds.d, norb, ctx.lds_max, tile_width(dtype),
+                                  stream);
+    });
+  }
+}
+
+// States per program of the multi-state path.
+constexpr int kMultiChunk = 256;
+
+// A batch of spin-balanced UCJ applications of one shape on the Givens path
+// with every state's tiles in the same launches: per merged rotation one
+// fiber launch per window pass of each sector over all states, per diagonal
+// term one column-factor launch and one elementwise launch over all states.
+// Each state goes through the same arithmetic as alone (ucj_impl), so the
+// results are bitwise those of the single-state entry.  Applies when the
+// beta rotation runs in place with strings on columns, the window holds a
+// full sub-window, and the states' rotations share one pass structure (as
+// rotations without vanishing Givens angles do); returns false otherwise,
+// before anything is launched.
+template <typename T>
+bool ucj_batched_multi(Context& ctx, int n_states, void* const* vecs, int dtype, int n, int na,
+                       int nb, int n_reps, const double* diag_coulomb_mats,
+                       const double* orbital_rotations, const double* final_rots,
+                       hipStream_t stream) {
+  constexpr int WW = std::is_same<T, double>::value ? 8 : 4;
+  const long da = (long)binom(n, na), db = (long)binom(n, nb);
+  if (n < kMaxSubWindow || n > kMaxDiagOrbitals || tile_width(dtype) != WW ||
+      !beta_in_columns(n, db, dtype, ctx))
+    return false;
+  const int w_a = choose_window(ctx, n, na, dtype), w_b = n;
+  if (w_a < kMaxSubWindow) return false;
+  if (dev_groups(ctx, w_a, dtype == FFSIM_C64).w2 != kMaxSubWindow ||
+      dev_groups(ctx, w_b, dtype == FFSIM_C64).w2 != kMaxSubWindow)
+    return false;
+  require(diag_grid(da, db).y <= 65535, FFSIM_ERR_VALUE, "too many columns");
+  const size_t nn = (size_t)n * n;
+  const int n_steps = n_reps + 1;
+  const bool same_order = triangle_order(n, w_a) == triangle_order(n, w_b);
+  for (int i0 = 0; i0 < n_states; i0 += kMultiChunk) {
+    const int m = std::min(kMultiChunk, n_states - i0);
+    // plan every state's rotations first: nothing is launched when they differ
+    std::vector<std::vector<SectorRotation>> ra(n_steps, std::vector<SectorRotation>(m)),
+        rb(n_steps, std::vector<SectorRotation>(m));
+    for (int i = 0; i < m; ++i) {
+      const int st = i0 + i;
+      std::vector<std::vector<cdouble>> V = ucj_merged_rotations(
+          n, n_reps,
+          reinterpret_cast<const cdouble*>(orbital_rotations) + (size_t)st * n_reps * nn,
+          final_rots ? reinterpret_cast<const cdouble*>(final_rots) + (size_t)st * nn : nullptr);
+      for (int r = 0; r < n_steps; ++r) {
+        ra[r][i] = prepare_rotation(V[r].data(), n, w_a);
+        rb[r][i] = same_order ? ra[r][i] : prepare_rotation(V[r].data(), n, w_b);
+      }
+    }
+    std::vector<MultiSector> ma(n_steps), mb(n_steps);
+    for (int r = 0; r < n_steps; ++r) {
+      std::vector<const SectorRotation*> pa(m), pb(m);
+      for (int i = 0; i < m; ++i) {
+        pa[i] = &ra[r][i];
+        pb[i] = &rb[r][i];
+      }
+      if (!plan_multi_sector(pa, n, w_a, ma[r]) || !plan_multi_sector(pb, n, w_b, mb[r])) {
+        if (i0 == 0) return false;
+        // earlier chunks are queued: the rest state by state (the beta sector
+        // runs in place, no work buffer)
+        for (int st = i0; st < n_states; ++st)
+          ucj_impl(ctx, vecs[st], nullptr, dtype, n, na, nb, n_reps,
+                   diag_coulomb_mats + (size_t)st * n_reps * 2 * nn,
+                   orbital_rotations + (size_t)st * n_reps * 2 * nn,
+                   final_rots ? final_rots + (size_t)st * 2 * nn : nullptr, stream);
+        return true;
+      }
+    }
+    Program prog(ctx, stream);
+    std::vector<const void*> ptrs(vecs + i0, vecs + i0 + m);
+    const size_t states_off = prog.append(ptrs.data(), m * sizeof(void*));
+    // the diagonal terms: headers, then coefficients, state-major per term
+    const DevStrings& sa = dev_strings(ctx, n, na);
+    const DevStrings& sb = dev_strings(ctx, n, nb);
+    std::vector<size_t> hoffs(n_reps), coffs(n_reps);
+    long coef_len = 0;
+    for (int r = 0; r < n_reps; ++r) {
+      std::vector<DiagHeader> hs(m);
+      std::vector<double> cs;
+      for (int i = 0; i < m; ++i) {
+        const double* J = diag_coulomb_mats + ((size_t)(i0 + i) * n_reps + r) * 2 * nn;
+        std::vector<double> halved(J, J + nn);
+        for (int k = 0; k < n; ++k) halved[(size_t)k * n + k] *= 0.5;
+        const DiagSpec spec = diag_coulomb_spec(n, halved.data(), J + nn, halved.data(), 0, -1.0);
+        hs[i] = diag_header(spec);
+        const std::vector<double> c = diag_coefficients(spec);
+        coef_len = (long)c.size();
+        cs.insert(cs.end(), c.begin(), c.end());
+      }
+      hoffs[r] = prog.append(hs.data(), hs.size() * sizeof(DiagHeader));
+      coffs[r] = prog.append(cs.data(), cs.size() * sizeof(double));
+    }
+    size_t goff = 0;
+    require(n_reps == 0 || prog.reserve((size_t)m * 2 * db * sizeof(double), &goff),
+            FFSIM_ERR_VALUE, "batched UCJ: column factors exceed the program scratch");
+    std::vector<std::function<void()>> launches;
+    for (int r = 0; r < n_steps; ++r) {
+      queue_multi_sector(ctx, prog, ma[r], dtype, MODE_ROWS, states_off, m, db, (int)db, n, na,
+                         w_a, stream, launches);
+      queue_multi_sector(ctx, prog, mb[r], dtype, MODE_COLS, states_off, m, db, (int)da, n, nb,
+                         w_b, stream, launches);
+      if (r == n_reps) break;
+      const size_t hoff = hoffs[r], coff = coffs[r];
+      launches.push_back([=, &prog, &sa, &sb]() {
+        const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
+        const double* dc = prog.dev<double>(coff);
+        double* dg = prog.scratch<double>(goff);
+        launch_diag_col_term(db, sb.d, dh, dc, dg, 1, stream, m, coef_len, 2 * db);
+        HIP_CHECK(hipGetLastError());
+        const double moved = 2.0 * sizeof(Cx<T>) * (double)da * (double)db * m;
+        const long pk = prof_start(PROF_DIAG, moved, stream);
+        launch_diag_pass_t<T>(DIAG_EVOLVE, nullptr, nullptr, da, db, sa.d, sb.d, dg, dh, dc,
+                              stream, prog.dev<void*>(states_off), m, coef_len, 2 * db);
+        HIP_CHECK(hipGetLastError());
+        prof_stop(pk, stream);
+      });
+    }
+    prog.commit();
+    for (auto& f : launches) f();
+    prog.finish();
+  }
+  return true;
+}
+
 constexpr long kMaxAuxStreams = 4;
 
 void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const* works,
@@ -1442,6 +1752,15 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
       ucj_batched_gemm<float>(ctx, n_states, vecs, works, n, na, nb, n_reps, diag_coulomb_mats,
                               orbital_rotations, final_rots, stream);
     return;
+  }
+  if (n_states >= 2 && batch_path_override() != 3) {
+    const bool done =
+        dtype == FFSIM_C128
+            ? ucj_batched_multi<double>(ctx, n_states, vecs, dtype, n, na, nb, n_reps,
+                                        diag_coulomb_mats, orbital_rotations, final_rots, stream)
+            : ucj_batched_multi<float>(ctx, n_states, vecs, dtype, n, na, nb, n_reps,
+                                       diag_coulomb_mats, orbital_rotations, final_rots, stream);
+    if (done) return;
   }
   // A launch over a small state has fewer tiles than the device has CUs, so
   // several states run at once, one per auxiliary stream, between events on

@@ -298,6 +298,18 @@ __host__ __device__ constexpr int fuse_nibble_entries(int win, int W) {
   return (((win + 3) >> 2) * 16 + 1) * W;
 }
 
+// Several states of one shape in one launch (the batched Givens path): tile t
+// covers fiber t / (n_states n_bt) of state (t / n_bt) % n_states, so the
+// tiles of one fiber stay together across states and share the staged
+// tables.  Each state is rotated in place at its own address, with its own
+// layer-pair coefficients and orbital phases at the given strides.
+struct MultiTiles {
+  void* const* states;  // device array of the states' addresses
+  int n_states;
+  long layer_stride;  // LayerPairs between consecutive states' coefficients
+  long diag_stride;   // doubles between consecutive states' phases
+};
+
 // Element (string i, batch j) of a fiber tile lives at
 //   MODE_ROWS: state[row_table[i] * ld + b0 + j]   (strings index rows)
 //   MODE_COLS: state[(b0 + j) * ld + row_table[i]] (strings index columns)
@@ -310,7 +322,7 @@ __host__ __device__ constexpr int fuse_nibble_entries(int win, int W) {
 // LDS, the next tile's loads are already in flight into registers.
 //
 // LDS: tile[max_rows * S] | rows[max_rows] | slots | desc | members | wave schedule
-template <typename T, int W, int MODE, int NTHR, int W2, bool FUSE = false>
+template <typename T, int W, int MODE, int NTHR, int W2, bool FUSE = false, bool MULTI = false>
 __global__ void __launch_bounds__(NTHR)
     fiber_kernel(const Cx<T>* src, Cx<T>* dst, long ld_src, long ld_dst, int n_batch, int tstore,
                  const Fiber* __restrict__ fibers, int n_fibers,
@@ -318,7 +330,7 @@ __global__ void __launch_bounds__(NTHR)
                  const SubPass* __restrict__ subs_g, int n_sub,
                  const LayerPair* __restrict__ layers_g, int n_layers,
                  const double* __restrict__ diag, const uint64_t* __restrict__ strings,
-                 int n_diag, int max_rows, int gd_cap, int gm_cap, DiagFuse fz) {
+                 int n_diag, int max_rows, int gd_cap, int gm_cap, DiagFuse fz, MultiTiles mt) {
   static_assert(!FUSE || MODE == MODE_ROWS, "the fused diagonal store is a row-mode store");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // LDS stride of one string's batch.  Row mode: unpadded (the register
@@ -363,7 +375,19 @@ __global__ void __launch_bounds__(NTHR)
 
   const int tid = threadIdx.x;
   const int n_bt = (n_batch + W - 1) / W;
-  const long n_tiles = (long)n_bt * n_fibers;
+  const int NS = MULTI ? mt.n_states : 1;
+  const long per_f = (long)n_bt * NS;  // tiles of one fiber (over all states)
+  const long n_tiles = per_f * n_fibers;
+  // the state of tile t and its address (one state: the src / dst arguments)
+  auto state_of = [&](long tt) -> int { return MULTI ? (int)((tt / n_bt) % NS) : 0; };
+  auto src_of = [&](int st) -> const Cx<T>* {
+    if constexpr (MULTI) return reinterpret_cast<const Cx<T>*>(mt.states[st]);
+    else return src;
+  };
+  auto dst_of = [&](int st) -> Cx<T>* {
+    if constexpr (MULTI) return reinterpret_cast<Cx<T>*>(mt.states[st]);
+    else return dst;
+  };
 
 
   // stage the row addresses and group tables of fiber f
@@ -431,7 +455,7 @@ __global__ void __launch_bounds__(NTHR)
   // and waiting for those (vmcnt) would also drain the stores in flight.
   // All row addresses are read first and the element loads issued after them, so
   // the global-table case waits once for its addresses instead of once per row.
-  auto load_regs = [&](Cx<T>* buf, int nr, int b0, int nb, const int32_t* rt) {
+  auto load_regs = [&](Cx<T>* buf, int nr, int b0, int nb, const int32_t* rt, const Cx<T>* sp) {
     int ra[CH];
     if (MODE == MODE_ROWS) {
       const int j = tid % W, i0 = tid / W;
@@ -443,7 +467,7 @@ __global__ void __launch_bounds__(NTHR)
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         buf[k] = Cx<T>{T(0), T(0)};
-        if (ra[k] >= 0) buf[k] = src[(long)ra[k] * ld_src + b0 + j];
+        if (ra[k] >= 0) buf[k] = sp[(long)ra[k] * ld_src + b0 + j];
       }
     } else {
 #pragma unroll
@@ -455,7 +479,7 @@ __global__ void __launch_bounds__(NTHR)
       for (int k = 0; k < CH; ++k) {
         const int j = k % W;
         buf[k] = Cx<T>{T(0), T(0)};
-        if (ra[k] >= 0) buf[k] = src[(long)(b0 + j) * ld_src + ra[k]];
+        if (ra[k] >= 0) buf[k] = sp[(long)(b0 + j) * ld_src + ra[k]];
       }
     }
   };
@@ -477,17 +501,17 @@ __global__ void __launch_bounds__(NTHR)
   };
   // rows [first, nr) straight from global memory into LDS
   constexpr int kCovered = (MODE == MODE_ROWS) ? CH * RSTEP : (CH / W) * NTHR;
-  auto load_rows = [&](int first, int nr, int b0, int nb) {
+  auto load_rows = [&](int first, int nr, int b0, int nb, const Cx<T>* sp) {
     if (nr <= first) return;
     if (MODE == MODE_ROWS) {
       const int j = tid % W, i0 = tid / W;
 #pragma unroll 4
       for (int i = first + i0; i < nr; i += RSTEP)
-        tile[i * S + j] = (j < nb) ? src[(long)rows[i] * ld_src + b0 + j] : Cx<T>{T(0), T(0)};
+        tile[i * S + j] = (j < nb) ? sp[(long)rows[i] * ld_src + b0 + j] : Cx<T>{T(0), T(0)};
     } else {
       for (int i = first + tid; i < nr; i += NTHR)
         for (int j = 0; j < W; ++j)
-          tile[i * S + j] = (j < nb) ? src[(long)(b0 + j) * ld_src + rows[i]] : Cx<T>{T(0), T(0)};
+          tile[i * S + j] = (j < nb) ? sp[(long)(b0 + j) * ld_src + rows[i]] : Cx<T>{T(0), T(0)};
     }
   };
   auto row_phase = [&](int i, T& pr, T& pim) {
@@ -504,7 +528,7 @@ __global__ void __launch_bounds__(NTHR)
     pr = (T)re;
     pim = (T)im;
   };
-  auto store_tile = [&](int nr, int b0, int nb) {
+  auto store_tile = [&](int nr, int b0, int nb, Cx<T>* dp) {
     if (MODE == MODE_ROWS && !tstore) {
       const int j = tid % W, i0 = tid / W;
       if (j < nb) {
@@ -518,7 +542,7 @@ __global__ void __launch_bounds__(NTHR)
             v.im = v.re * pim + v.im * pr;
             v.re = t;
           }
-          dst[(long)rows[i] * ld_dst + b0 + j] = v;
+          dp[(long)rows[i] * ld_dst + b0 + j] = v;
         }
       }
     } else if (FUSE) {
@@ -574,7 +598,7 @@ __global__ void __launch_bounds__(NTHR)
             const T t = v.re * (T)P[j].re - v.im * (T)P[j].im;
             v.im = v.re * (T)P[j].im + v.im * (T)P[j].re;
             v.re = t;
-            dst[(long)(b0 + j) * ld_dst + r] = v;
+            dp[(long)(b0 + j) * ld_dst + r] = v;
           }
         }
       }
@@ -592,7 +616,7 @@ __global__ void __launch_bounds__(NTHR)
               v.im = v.re * pim + v.im * pr;
               v.re = t;
             }
-            dst[(long)(b0 + j) * ld_dst + r] = v;
+            dp[(long)(b0 + j) * ld_dst + r] = v;
           }
         }
       }
@@ -629,8 +653,9 @@ __global__ void __launch_bounds__(NTHR)
   long t = bid;
   if (t >= n_tiles) return;
   if (diag)
-    for (int k = tid; k < 2 * n_diag; k += NTHR) dph[k] = diag[k];
-  int f = (int)(t / n_bt);
+    for (int k = tid; k < 2 * n_diag; k += NTHR)
+      dph[k] = diag[(MULTI ? (long)state_of(t) * mt.diag_stride : 0) + k];
+  int f = (int)(t / per_f);
   Fiber fd = fibers[f];
   stage_fiber(fd, true);
   __syncthreads();
@@ -639,7 +664,7 @@ __global__ void __launch_bounds__(NTHR)
   {
     const int b0 = (int)(t % n_bt) * W;
     const int nb = min(W, n_batch - b0);
-    load_rows(0, fd.n_rows, b0, nb);
+    load_rows(0, fd.n_rows, b0, nb, src_of(state_of(t)));
     if (FUSE) bfac_to_lds(load_bfac(b0), b0);
   }
   while (true) {
@@ -647,7 +672,9 @@ __global__ void __launch_bounds__(NTHR)
     const int b0 = (int)(t % n_bt) * W;
     const int nb = min(W, n_batch - b0);
     const long tn = t + gridDim.x;
-    const int fn = tn < n_tiles ? (int)(tn / n_bt) : -1;
+    const int fn = tn < n_tiles ? (int)(tn / per_f) : -1;
+    const int st = state_of(t);
+    const int stn = tn < n_tiles ? state_of(tn) : st;
     const bool same_fiber = (fn == f);
     // the next tile is prefetched also when it starts a new fiber: its rows then
     // come from the global row table (one more dependent load, hidden by the
@@ -661,9 +688,9 @@ __global__ void __launch_bounds__(NTHR)
     // around the loop: a loop-carried buf makes the compiler copy it at the back
     // edge behind a vmcnt(0) that would also drain this tile's stores
     if (same_fiber)
-      load_regs(buf, nrn, b0n, nbn, rows);
+      load_regs(buf, nrn, b0n, nbn, rows, src_of(stn));
     else
-      load_regs(buf, nrn, b0n, nbn, row_table + fdn.row_off);
+      load_regs(buf, nrn, b0n, nbn, row_table + fdn.row_off, src_of(stn));
     if (FUSE) bfn = load_bfac(b0n);
 
     // ---- register sub-passes on the current tile
@@ -674,7 +701,7 @@ __global__ void __launch_bounds__(NTHR)
       // the pass program is uniform: scalar loads straight from device memory
       const SubPass P = subs_g[sp];
       const GroupSlot gs = gslot[P.slot];
-      const LayerPair* lp = layers_g + P.lp0;
+      const LayerPair* lp = layers_g + (MULTI ? (long)st * mt.layer_stride : 0) + P.lp0;
       const int c0 = wsched[(P.slot * NW + wave) * 2];
       const int nc = wsched[(P.slot * NW + wave) * 2 + 1];
       for (int c = 0; c < nc; ++c) {
@@ -696,9 +723,13 @@ __global__ void __launch_bounds__(NTHR)
     // (vmcnt counts stores too: a wait placed after them would drain them).
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt/lgkmcnt unchanged
     __syncthreads();
-    store_tile(nr, b0, nb);
+    store_tile(nr, b0, nb, dst_of(st));
     __syncthreads();
     if (tn >= n_tiles) break;
+    // the next tile's state: its orbital phases (read by the next store, after
+    // several barriers)
+    if (MULTI && diag && stn != st)
+      for (int k = tid; k < 2 * n_diag; k += NTHR) dph[k] = diag[(long)stn * mt.diag_stride + k];
     if (!same_fiber) {
       f = fn;
       const bool tables = fdn.kp != fd.kp;
@@ -708,7 +739,7 @@ __global__ void __launch_bounds__(NTHR)
     }
     regs_to_lds(buf, nrn);
     if (FUSE) bfac_to_lds(bfn, b0n);
-    load_rows(kCovered, nrn, b0n, nbn);
+    load_rows(kCovered, nrn, b0n, nbn, src_of(stn));
     t = tn;
   }
 }
@@ -730,11 +761,13 @@ enum { DF_QUAD = 1, DF_CROSS = 2, DF_LIN = 4 };
 
 // Column term gamma[c] (quadratic + linear) for every column string.  For
 // evolution it writes the column's phase factor exp(-i scale gamma[c]) as
-// (cos, sin) pairs instead.
+// (cos, sin) pairs instead.  grid.y runs over operators (the states of a
+// batch): operator y has header hdr[y], coefficients coef + y coef_stride and
+// writes gamma + y gamma_stride.
 __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ col_str,
                                      const DiagHeader* __restrict__ hdr,
                                      const double* __restrict__ coef, double* __restrict__ gamma,
-                                     int evolve);
+                                     int evolve, long coef_stride, long gamma_stride);
 
 // One pass of a diagonal operator over an R-row x n_cols_blk-column block of
 // the state.  E(a, b) = rho_a + gamma_b + cross(a, b), cross = sum over the
@@ -751,12 +784,26 @@ __global__ void diag_col_term_kernel(int n_cols, const uint64_t* __restrict__ co
 // in flight while the current one is formed and stored; loads read a clamped
 // column (always in bounds) and only stores are guarded, so that the memory
 // counters never wait on a branch.
+//
+// grid.z runs over the states of a batch when `states` is given: state z is
+// transformed in place at states[z] by operator z (header hdr[z],
+// coefficients coef + z coef_stride, column values colv + z colv_stride), the
+// same arithmetic as a launch for that state alone.
 template <typename T, int R, int MODE>
 __global__ void __launch_bounds__(256)
     diag_kernel(const Cx<T>* src, Cx<T>* dst, long ld, int n_rows,
                 int n_cols, int n_cols_blk, const uint64_t* __restrict__ row_str,
                 const uint64_t* __restrict__ col_str, const double* __restrict__ colv,
-                const DiagHeader* __restrict__ hdr, const double* __restrict__ coef) {
+                const DiagHeader* __restrict__ hdr, const double* __restrict__ coef,
+                Cx<T>* const* __restrict__ states, long coef_stride, long colv_stride) {
+  if (states) {
+    const int zs = blockIdx.z;
+    src = states[zs];
+    dst = states[zs];
+    hdr += zs;
+    coef += zs * coef_stride;
+    if (colv) colv += zs * colv_stride;
+  }
   constexpr int NT = 256;
   constexpr int LPR = NT / R;  // lanes per row in the row-term reduction (<= 64)
   static_assert(LPR <= 64 && (LPR & (LPR - 1)) == 0, "R must be a power of two >= 4");

@@ -148,13 +148,16 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
  * taken).  diag_coulomb_mats: n_states x n_reps x 2 x norb x norb;
  * orbital_rotations: n_states x n_reps x norb x norb complex128;
  * final_orbital_rotations: NULL or n_states x norb x norb.  Small states
- * (at most 6 electrons per sector and at most 100 strings, or 256 strings in
+ * (at most 6 electrons per sector and at most 100 strings, or 150 strings in
  * batches of 8 or more) take a GEMM path: each merged orbital rotation is the
  * compound matrix Lambda^k(U) (built on the device) applied to all states by
- * batched complex GEMMs, and each diagonal term one batched elementwise pass;
- * other states run the Givens path state by state on the stream (on the
- * default stream, small states run on up to 4 library-owned streams at once,
- * between events on the default stream). */
+ * batched complex GEMMs, and each diagonal term one batched elementwise pass.
+ * Other states take the Givens path.  Where a beta vector fits one tile
+ * (norb <= 12 in complex128, <= 14 in complex64) and norb >= 6, every window
+ * pass and diagonal term is one launch over all states, with results bitwise
+ * equal to the single-state entry's; otherwise the states run one by one on
+ * the stream (on the default stream, small states run on up to 4
+ * library-owned streams at once, between events on the default stream). */
 int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void* const* works,
                                           int dtype, int norb, int n_alpha, int n_beta,
                                           int n_reps, const double* diag_coulomb_mats,

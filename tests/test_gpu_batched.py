@@ -4,9 +4,11 @@ ffsim_apply_ucj_spin_balanced_batched applies a different operator to each
 state of a batch of one shape.  Small states take the compound-matrix GEMM
 path (psi <- Lambda^k(U_a) psi Lambda^k(U_b)^T with the merged rotations of
 python/ffsim/variational/ucj_spin_balanced.py:657-696, batched diagonal
-passes); larger ones the Givens path state by state.  Each result is compared
-with the oracle's UCJ application of its own operator (complex128 1e-12,
-complex64 2e-6), and with the single-state entry point.
+passes); larger ones the Givens path, with the tiles of all states in the
+same launches where the beta sector fits a tile (else state by state).  Each
+result is compared with the oracle's UCJ application of its own operator
+(complex128 1e-12, complex64 2e-6), and with the single-state entry point,
+which the Givens path matches bit for bit.
 """
 
 from __future__ import annotations
@@ -31,7 +33,7 @@ def _pairs(norb):
     return [(p, p + 1) for p in range(norb - 1)], [(p, p) for p in range(norb)]
 
 
-@pytest.mark.parametrize("path", ["auto", "gemm", "sweep"])
+@pytest.mark.parametrize("path", ["auto", "gemm", "sweep", "states"])
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize(
     "norb,nelec,n_states,final",
@@ -40,7 +42,9 @@ def _pairs(norb):
 )
 def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype, path, monkeypatch):
     """path: the library's choice, or FFSIM_AMD_BATCH_PATH forcing the GEMM path
-    (taken up to 1024 strings per sector) or the Givens path."""
+    (taken up to 1024 strings per sector), the Givens path (all states in one
+    launch per window pass where that applies) or the Givens path state by
+    state."""
     import torch
 
     import ffsim_amd
@@ -62,7 +66,7 @@ def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype, path, m
     for op, s in zip(ops, single):
         op.apply_in_place(s, norb, nelec)
     tol = 1e-12 if dtype == "c128" else 2e-6
-    if path == "sweep":
+    if path in ("sweep", "states"):
         # the Givens path runs each state exactly as the single-state entry does
         for got, one in zip(states, single):
             assert torch.equal(got, one)
@@ -87,3 +91,46 @@ def test_batched_ucj_rejects_mixed_operators():
     states = [torch.zeros(36, dtype=torch.complex128, device="cuda") for _ in range(2)]
     with pytest.raises(ValueError, match="n_reps"):
         UCJOpSpinBalanced.apply_batched_in_place([a, b], states, norb, nelec)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_batched_multi_state_mixed_structures(dtype, monkeypatch):
+    """Over 256 states (two programs of the multi-state path), with a state
+    whose merged rotation is the identity (every Givens angle vanishes, so its
+    passes differ from the others') in the first program of one batch and in
+    the second of another: the first batch runs state by state, the second
+    queues its first program and runs the rest state by state.  Every result
+    equals the single-state entry's bit for bit."""
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd.variational.ucj_spin_balanced import UCJOpSpinBalanced
+
+    norb, nelec, n_states = 6, (3, 2), 300
+    monkeypatch.setenv("FFSIM_AMD_BATCH_PATH", "sweep")
+    rng = np.random.default_rng(7)
+    cdt = torch.complex128 if dtype == "c128" else torch.complex64
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    for odd in (3, 280):
+        _case("batched ucj mixed structures", norb, nelec, n_states, odd, dtype)
+        ops = [ffsim_amd.random.random_ucj_op_spin_balanced(
+            norb, n_reps=2, interaction_pairs=_pairs(norb), seed=rng) for _ in range(n_states)]
+        rots = ops[odd].orbital_rotations.copy()
+        rots[1] = rots[0]  # merged rotation U_1^H U_0 = 1
+        ops[odd] = UCJOpSpinBalanced(diag_coulomb_mats=ops[odd].diag_coulomb_mats,
+                                     orbital_rotations=rots)
+        vecs = [O.random_state_vector(dim, rng) for _ in range(n_states)]
+        states = [torch.from_numpy(v).to("cuda").to(cdt) for v in vecs]
+        single = [s.clone() for s in states]
+        UCJOpSpinBalanced.apply_batched_in_place(ops, states, norb, nelec)
+        for op, s in zip(ops, single):
+            op.apply_in_place(s, norb, nelec)
+        for got, one in zip(states, single):
+            assert torch.equal(got, one)
+        tol = 1e-12 if dtype == "c128" else 2e-6
+        for k in (0, odd, n_states - 1):
+            op = ops[k]
+            expected = O.apply_ucj_spin_balanced(vecs[k], op.diag_coulomb_mats,
+                                                 op.orbital_rotations, None, norb, nelec)
+            np.testing.assert_allclose(states[k].cpu().numpy().astype(np.complex128), expected,
+                                       atol=tol, rtol=0)

@@ -1077,19 +1077,34 @@ void run_sector(Context& ctx, Program& prog, int dtype, int mode, void* state, l
   }
 }
 
+// Row pitch (elements) of the transposed state in the work buffer: dim_a
+// rounded up to 32 bytes, so that every row's tile segments start on the
+// 32-byte boundaries of the memory requests.  (At norb=16 a complex64 row of
+// 12870 elements is 16 bytes off on every other row, and an unaligned 32-byte
+// segment costs two requests.)
+long transposed_ld(int dtype, long dim_a) {
+  const long q = 32 / (long)elem_bytes(dtype);
+  return (dim_a + q - 1) / q * q;
+}
+
 // Full spinful orbital rotation (both sectors) on a dim_a x dim_b state.
 // With `then_diag`, the diagonal Coulomb evolution that follows the rotation is
 // fused into the final pass when that pass is a transposing c128 row pass and
 // everything fits; returns true when it was (the caller applies it otherwise).
+// ld_w: row pitch of the transposed state in `work` (0: transposed_ld, the
+// layout ffsim_orbital_rotation_work_elements sizes the buffer for).
 bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int norb, int na,
                            int nb, const cdouble* mat_a, const cdouble* mat_b,
-                           hipStream_t stream, const DiagSpec* then_diag = nullptr) {
+                           hipStream_t stream, const DiagSpec* then_diag = nullptr,
+                           long ld_w = 0) {
   const long dim_a = (long)binom(norb, na), dim_b = (long)binom(norb, nb);
   if (dim_a == 0 || dim_b == 0 || norb == 0) return false;
   require(dim_a < (1l << 31) && dim_b < (1l << 31), FFSIM_ERR_VALUE, "dimension too large");
   const bool beta_cols = beta_in_columns(norb, dim_b, dtype, ctx);
   const int w_a = choose_window(ctx, norb, na, dtype);
   const int w_b = beta_cols ? norb : choose_window(ctx, norb, nb, dtype);
+  if (ld_w == 0) ld_w = transposed_ld(dtype, dim_a);
+  require(ld_w >= dim_a, FFSIM_ERR_VALUE, "work row pitch below dim_a");
   SectorRotation ra, rb;
   if (mat_a) ra = prepare_rotation(mat_a, norb, w_a);
   if (mat_b) {
@@ -1137,14 +1152,14 @@ bool orbital_rotation_impl(Context& ctx, void* vec, void* work, int dtype, int n
     SectorRotation empty;
     const SectorRotation& r = mat_a ? ra : empty;
     run_sector(ctx, prog, dtype, MODE_ROWS, vec, dim_b, (int)dim_b, norb, na, r,
-               beta_transposed ? work : nullptr, dim_a, stream, launches);
+               beta_transposed ? work : nullptr, ld_w, stream, launches);
   }
   if (mat_b) {
     if (beta_cols) {
       run_sector(ctx, prog, dtype, MODE_COLS, vec, dim_b, (int)dim_a, norb, nb, rb, nullptr, 0,
                  stream, launches);
     } else {
-      run_sector(ctx, prog, dtype, MODE_ROWS, work, dim_a, (int)dim_a, norb, nb, rb, vec, dim_b,
+      run_sector(ctx, prog, dtype, MODE_ROWS, work, ld_w, (int)dim_a, norb, nb, rb, vec, dim_b,
                  stream, launches, want_fuse ? &ft : nullptr, &fused);
     }
   }
@@ -1208,18 +1223,19 @@ void check_sizes(int norb, int na, int nb) {
 void diag_coulomb_sequence_impl(Context& ctx, void* vec, void* work, int dtype, int norb, int na,
                                 int nb, int n_terms, const cdouble* const* rots,
                                 const double* const* mats /* aa, ab, bb per term (3 ptrs) */,
-                                const double* times, int z_rep, hipStream_t st) {
+                                const double* times, int z_rep, hipStream_t st, long ld_w = 0) {
   for (int t = 0; t < n_terms; ++t) {
     DiagSpec s = diag_coulomb_spec(norb, mats[3 * t], mats[3 * t + 1], mats[3 * t + 2], z_rep,
                                    times[t]);
     bool fused = false;
     if (rots[t])
       fused = orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[t], rots[t], st,
-                                    s.flags ? &s : nullptr);
+                                    s.flags ? &s : nullptr, ld_w);
     if (s.flags && !fused) diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, norb, na, nb, s, st);
   }
   if (rots[n_terms])
-    orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[n_terms], rots[n_terms], st);
+    orbital_rotation_impl(ctx, vec, work, dtype, norb, na, nb, rots[n_terms], rots[n_terms], st,
+                          nullptr, ld_w);
 }
 
 std::vector<cdouble> matmul(const cdouble* A, const cdouble* B, int n, bool adj_a) {
@@ -1295,9 +1311,10 @@ std::vector<std::vector<cdouble>> ucj_merged_rotations(int n, int n_reps, const 
   return V;
 }
 
+// ld_w: row pitch of the transposed state in `work` (0: transposed_ld).
 void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int nb, int n_reps,
               const double* diag_coulomb_mats, const double* orbital_rotations,
-              const double* final_orbital_rotation, hipStream_t stream) {
+              const double* final_orbital_rotation, hipStream_t stream, long ld_w = 0) {
   const size_t nn = (size_t)n * n;
   std::vector<std::vector<cdouble>> V = ucj_merged_rotations(
       n, n_reps, reinterpret_cast<const cdouble*>(orbital_rotations),
@@ -1316,7 +1333,7 @@ void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int
     mats[3 * r + 2] = halved[r].data();
   }
   diag_coulomb_sequence_impl(ctx, vec, work, dtype, n, na, nb, n_reps, rots.data(), mats.data(),
-                             times.data(), 0, stream);
+                             times.data(), 0, stream, ld_w);
 }
 
 // Where the GEMM path is taken by default.  An orbital rotation there costs
@@ -1778,7 +1795,7 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
       ucj_impl(ctx, vecs[i], works[i], dtype, n, na, nb, n_reps,
                diag_coulomb_mats + (size_t)i * n_reps * 2 * nn,
                orbital_rotations + (size_t)i * n_reps * 2 * nn,
-               final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, stream);
+               final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, stream, da);
     return;
   }
   {
@@ -1801,7 +1818,7 @@ void ucj_batched_impl(Context& ctx, int n_states, void* const* vecs, void* const
     ucj_impl(ctx, vecs[i], works[i], dtype, n, na, nb, n_reps,
              diag_coulomb_mats + (size_t)i * n_reps * 2 * nn,
              orbital_rotations + (size_t)i * n_reps * 2 * nn,
-             final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, ctx.aux[i % n_aux]);
+             final_rots ? final_rots + (size_t)i * 2 * nn : nullptr, ctx.aux[i % n_aux], da);
   for (int k = 0; k < n_aux; ++k) {
     hipEvent_t done = event();
     HIP_CHECK(hipEventRecord(done, ctx.aux[k]));
@@ -1955,7 +1972,7 @@ int64_t ffsim_orbital_rotation_work_elements(int dtype, int norb, int n_alpha, i
     Context& ctx = context();
     const long dim_a = (long)binom(norb, n_alpha), dim_b = (long)binom(norb, n_beta);
     const bool beta_cols = beta_in_columns(norb, dim_b, dtype, ctx);
-    out = beta_cols ? 0 : (int64_t)dim_a * dim_b;
+    out = beta_cols ? 0 : (int64_t)transposed_ld(dtype, dim_a) * dim_b;
   });
   return rc == FFSIM_OK ? out : rc;
 }

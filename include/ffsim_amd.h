@@ -71,11 +71,13 @@ int ffsim_givens_decomposition(const double* mat, int n, double tol, double* c, 
 
 /* ---- device gate application ------------------------------------------ */
 
-/* Number of state elements the `work` buffer of ffsim_apply_orbital_rotation
- * and ffsim_apply_ucj_spin_balanced needs for this shape (0: none, NULL may
- * be passed).  Negative on error.  No reference counterpart: it sizes the
- * transposition buffer that replaces the reference's transpose copies
- * (python/ffsim/gates/orbital_rotation.py:143, :152). */
+/* Number of state elements the `work` buffer of ffsim_apply_orbital_rotation,
+ * ffsim_apply_ucj_spin_balanced and ffsim_apply_diag_coulomb_sequence needs
+ * for this shape (0: none, NULL may be passed).  Negative on error.  No
+ * reference counterpart: it sizes the transposition buffer that replaces the
+ * reference's transpose copies (python/ffsim/gates/orbital_rotation.py:143,
+ * :152); the transposed state's rows are padded to 32 bytes, so this can
+ * exceed dim_a * dim_b by a few elements per beta string. */
 int64_t ffsim_orbital_rotation_work_elements(int dtype, int norb, int n_alpha, int n_beta);
 
 /* Replaces the spinful path of ffsim.apply_orbital_rotation

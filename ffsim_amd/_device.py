@@ -165,10 +165,15 @@ def check_size(t: torch.Tensor, norb: int, nelec) -> None:
         )
 
 
-def work_buffer(t: torch.Tensor, norb: int, na: int, nb: int) -> torch.Tensor | None:
-    n = _lib.check(
+def work_elements(t: torch.Tensor, norb: int, na: int, nb: int) -> int:
+    """Elements of the transposition buffer an orbital rotation of t needs (0: none)."""
+    return _lib.check(
         _lib.lib().ffsim_orbital_rotation_work_elements(dtype_tag(t), norb, na, nb)
     )
+
+
+def work_buffer(t: torch.Tensor, norb: int, na: int, nb: int) -> torch.Tensor | None:
+    n = work_elements(t, norb, na, nb)
     if n == 0:
         return None
     return torch.empty(n, dtype=t.dtype, device=t.device)

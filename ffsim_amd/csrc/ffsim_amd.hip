@@ -1630,6 +1630,72 @@ void queue_multi_sector(Context& ctx, Program& prog, const MultiSector& ms, int 
 // States per program of the multi-state path.
 constexpr int kMultiChunk = 256;
 
+// The window widths of the multi-state path for this shape; false where it
+// does not apply: a beta vector must fit one tile (the beta sector runs in
+// place with strings on columns, its window all norb orbitals), both windows
+// must hold a full register sub-window, and the tiles must have the
+// production width.
+bool multi_windows(Context& ctx, int dtype, int n, int na, int nb, int& w_a, int& w_b) {
+  const int ww = dtype == FFSIM_C128 ? 8 : 4;
+  const long db = (long)binom(n, nb);
+  if (n < kMaxSubWindow || n > kMaxDiagOrbitals || tile_width(dtype) != ww ||
+      !beta_in_columns(n, db, dtype, ctx))
+    return false;
+  w_a = choose_window(ctx, n, na, dtype);
+  w_b = n;
+  if (w_a < kMaxSubWindow) return false;
+  return dev_groups(ctx, w_a, dtype == FFSIM_C64).w2 == kMaxSubWindow &&
+         dev_groups(ctx, w_b, dtype == FFSIM_C64).w2 == kMaxSubWindow;
+}
+
+// One orbital rotation per state, planned for multi-state passes.
+struct MultiRotation {
+  MultiSector a, b;
+  bool has_a = false, has_b = false;
+};
+
+// mats_a[i], mats_b[i]: state i's matrices; a sector whose first entry is null
+// is not rotated (then every entry is null).  The beta sector reuses the
+// alpha decomposition where ffsim_apply_orbital_rotation does.  False when
+// the states' pass structures differ.
+bool plan_multi_rotation(int n, int w_a, int w_b, const std::vector<const cdouble*>& mats_a,
+                         const std::vector<const cdouble*>& mats_b, MultiRotation& mr) {
+  const size_t m = mats_a.size();
+  mr.has_a = mats_a[0] != nullptr;
+  mr.has_b = mats_b[0] != nullptr;
+  const bool same_order = triangle_order(n, w_a) == triangle_order(n, w_b);
+  std::vector<SectorRotation> ra(m), rb(m);
+  std::vector<const SectorRotation*> pa(m), pb(m);
+  for (size_t i = 0; i < m; ++i) {
+    if (mr.has_a) ra[i] = prepare_rotation(mats_a[i], n, w_a);
+    if (mr.has_b) {
+      if (mr.has_a && same_order &&
+          std::memcmp(mats_a[i], mats_b[i], sizeof(cdouble) * n * n) == 0)
+        rb[i] = ra[i];
+      else
+        rb[i] = prepare_rotation(mats_b[i], n, w_b);
+    }
+    pa[i] = &ra[i];
+    pb[i] = &rb[i];
+  }
+  return (!mr.has_a || plan_multi_sector(pa, n, w_a, mr.a)) &&
+         (!mr.has_b || plan_multi_sector(pb, n, w_b, mr.b));
+}
+
+// Queues a planned multi-state rotation: the alpha sector's passes (strings on
+// rows), then the beta sector's (strings on columns), in place.
+void queue_multi_rotation(Context& ctx, Program& prog, const MultiRotation& mr, int dtype,
+                          size_t states_off, int m, int n, int na, int nb, int w_a, int w_b,
+                          hipStream_t stream, std::vector<std::function<void()>>& launches) {
+  const long da = (long)binom(n, na), db = (long)binom(n, nb);
+  if (mr.has_a)
+    queue_multi_sector(ctx, prog, mr.a, dtype, MODE_ROWS, states_off, m, db, (int)db, n, na, w_a,
+                       stream, launches);
+  if (mr.has_b)
+    queue_multi_sector(ctx, prog, mr.b, dtype, MODE_COLS, states_off, m, db, (int)da, n, nb, w_b,
+                       stream, launches);
+}
+
 // A batch of spin-balanced UCJ applications of one shape on the Givens path
 // with every state's tiles in the same launches: per merged rotation one
 // fiber launch per window pass of each sector over all states, per diagonal
@@ -1645,44 +1711,28 @@ bool ucj_batched_multi(Context& ctx, int n_states, void* const* vecs, int dtype,
                        int nb, int n_reps, const double* diag_coulomb_mats,
                        const double* orbital_rotations, const double* final_rots,
                        hipStream_t stream) {
-  constexpr int WW = std::is_same<T, double>::value ? 8 : 4;
   const long da = (long)binom(n, na), db = (long)binom(n, nb);
-  if (n < kMaxSubWindow || n > kMaxDiagOrbitals || tile_width(dtype) != WW ||
-      !beta_in_columns(n, db, dtype, ctx))
-    return false;
-  const int w_a = choose_window(ctx, n, na, dtype), w_b = n;
-  if (w_a < kMaxSubWindow) return false;
-  if (dev_groups(ctx, w_a, dtype == FFSIM_C64).w2 != kMaxSubWindow ||
-      dev_groups(ctx, w_b, dtype == FFSIM_C64).w2 != kMaxSubWindow)
-    return false;
+  int w_a = 0, w_b = 0;
+  if (!multi_windows(ctx, dtype, n, na, nb, w_a, w_b)) return false;
   require(diag_grid(da, db).y <= 65535, FFSIM_ERR_VALUE, "too many columns");
   const size_t nn = (size_t)n * n;
   const int n_steps = n_reps + 1;
-  const bool same_order = triangle_order(n, w_a) == triangle_order(n, w_b);
   for (int i0 = 0; i0 < n_states; i0 += kMultiChunk) {
     const int m = std::min(kMultiChunk, n_states - i0);
     // plan every state's rotations first: nothing is launched when they differ
-    std::vector<std::vector<SectorRotation>> ra(n_steps, std::vector<SectorRotation>(m)),
-        rb(n_steps, std::vector<SectorRotation>(m));
+    std::vector<std::vector<std::vector<cdouble>>> V(m);
     for (int i = 0; i < m; ++i) {
       const int st = i0 + i;
-      std::vector<std::vector<cdouble>> V = ucj_merged_rotations(
+      V[i] = ucj_merged_rotations(
           n, n_reps,
           reinterpret_cast<const cdouble*>(orbital_rotations) + (size_t)st * n_reps * nn,
           final_rots ? reinterpret_cast<const cdouble*>(final_rots) + (size_t)st * nn : nullptr);
-      for (int r = 0; r < n_steps; ++r) {
-        ra[r][i] = prepare_rotation(V[r].data(), n, w_a);
-        rb[r][i] = same_order ? ra[r][i] : prepare_rotation(V[r].data(), n, w_b);
-      }
     }
-    std::vector<MultiSector> ma(n_steps), mb(n_steps);
+    std::vector<MultiRotation> mr(n_steps);
     for (int r = 0; r < n_steps; ++r) {
-      std::vector<const SectorRotation*> pa(m), pb(m);
-      for (int i = 0; i < m; ++i) {
-        pa[i] = &ra[r][i];
-        pb[i] = &rb[r][i];
-      }
-      if (!plan_multi_sector(pa, n, w_a, ma[r]) || !plan_multi_sector(pb, n, w_b, mb[r])) {
+      std::vector<const cdouble*> pm(m);
+      for (int i = 0; i < m; ++i) pm[i] = V[i][r].data();
+      if (!plan_multi_rotation(n, w_a, w_b, pm, pm, mr[r])) {
         if (i0 == 0) return false;
         // earlier chunks are queued: the rest state by state (the beta sector
         // runs in place, no work buffer)
@@ -1723,10 +1773,8 @@ bool ucj_batched_multi(Context& ctx, int n_states, void* const* vecs, int dtype,
             FFSIM_ERR_VALUE, "batched UCJ: column factors exceed the program scratch");
     std::vector<std::function<void()>> launches;
     for (int r = 0; r < n_steps; ++r) {
-      queue_multi_sector(ctx, prog, ma[r], dtype, MODE_ROWS, states_off, m, db, (int)db, n, na,
-                         w_a, stream, launches);
-      queue_multi_sector(ctx, prog, mb[r], dtype, MODE_COLS, states_off, m, db, (int)da, n, nb,
-                         w_b, stream, launches);
+      queue_multi_rotation(ctx, prog, mr[r], dtype, states_off, m, n, na, nb, w_a, w_b, stream,
+                           launches);
       if (r == n_reps) break;
       const size_t hoff = hoffs[r], coff = coffs[r];
       launches.push_back([=, &prog, &sa, &sb]() {
@@ -1748,6 +1796,47 @@ bool ucj_batched_multi(Context& ctx, int n_states, void* const* vecs, int dtype,
     prog.finish();
   }
   return true;
+}
+
+// Orbital rotations of a batch of states of one shape (mats_a / mats_b: null,
+// or one norb x norb matrix per state): where the multi-state path applies,
+// one launch per window pass and sector over all states; otherwise (or when
+// the states' pass structures differ) state by state on the stream, so the
+// states' work buffers may be one buffer.  Either way every state's result is
+// bitwise what ffsim_apply_orbital_rotation gives.
+void orbital_rotation_batched_impl(Context& ctx, int n_states, void* const* vecs,
+                                   void* const* works, int dtype, int n, int na, int nb,
+                                   const cdouble* mats_a, const cdouble* mats_b,
+                                   hipStream_t stream) {
+  const long da = (long)binom(n, na), db = (long)binom(n, nb);
+  if (da == 0 || db == 0 || n == 0 || (!mats_a && !mats_b)) return;
+  const size_t nn = (size_t)n * n;
+  int w_a = 0, w_b = 0;
+  const bool multi = n_states >= 2 && multi_windows(ctx, dtype, n, na, nb, w_a, w_b);
+  for (int i0 = 0; i0 < n_states; i0 += kMultiChunk) {
+    const int m = std::min(kMultiChunk, n_states - i0);
+    std::vector<const cdouble*> pa(m), pb(m);
+    for (int i = 0; i < m; ++i) {
+      pa[i] = mats_a ? mats_a + (size_t)(i0 + i) * nn : nullptr;
+      pb[i] = mats_b ? mats_b + (size_t)(i0 + i) * nn : nullptr;
+    }
+    MultiRotation mr;
+    if (multi && m >= 2 && plan_multi_rotation(n, w_a, w_b, pa, pb, mr)) {
+      Program prog(ctx, stream);
+      std::vector<const void*> ptrs(vecs + i0, vecs + i0 + m);
+      const size_t states_off = prog.append(ptrs.data(), m * sizeof(void*));
+      std::vector<std::function<void()>> launches;
+      queue_multi_rotation(ctx, prog, mr, dtype, states_off, m, n, na, nb, w_a, w_b, stream,
+                           launches);
+      prog.commit();
+      for (auto& f : launches) f();
+      prog.finish();
+    } else {
+      for (int i = 0; i < m; ++i)
+        orbital_rotation_impl(ctx, vecs[i0 + i], works ? works[i0 + i] : nullptr, dtype, n, na,
+                              nb, pa[i], pb[i], stream);
+    }
+  }
 }
 
 constexpr long kMaxAuxStreams = 4;
@@ -1987,6 +2076,25 @@ int ffsim_apply_orbital_rotation(void* vec, void* work, int dtype, int norb, int
     orbital_rotation_impl(ctx, vec, work, dtype, norb, n_alpha, n_beta,
                           reinterpret_cast<const cdouble*>(mat_a),
                           reinterpret_cast<const cdouble*>(mat_b), (hipStream_t)stream);
+  });
+}
+
+int ffsim_apply_orbital_rotation_batched(int n_states, void* const* vecs, void* const* works,
+                                         int dtype, int norb, int n_alpha, int n_beta,
+                                         const double* mats_a, const double* mats_b,
+                                         void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(n_states >= 0, FFSIM_ERR_VALUE, "negative batch size");
+    if (n_states == 0) return;
+    require(vecs != nullptr, FFSIM_ERR_VALUE, "batched rotation: no state array");
+    for (int i = 0; i < n_states; ++i)
+      require(vecs[i] != nullptr, FFSIM_ERR_VALUE, "batched rotation: null state");
+    Context& ctx = context();
+    orbital_rotation_batched_impl(ctx, n_states, vecs, works, dtype, norb, n_alpha, n_beta,
+                                  reinterpret_cast<const cdouble*>(mats_a),
+                                  reinterpret_cast<const cdouble*>(mats_b), (hipStream_t)stream);
   });
 }
 

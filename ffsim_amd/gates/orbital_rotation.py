@@ -9,6 +9,8 @@ network in fused window passes on the GPU.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -50,6 +52,40 @@ def orbital_rotation_in_place(t: torch.Tensor, mat, norb: int, nelec) -> torch.T
         _device.stream_ptr(),
     )
     return t
+
+
+def orbital_rotations_in_place(states, mats, norb: int, nelec: tuple) -> list:
+    """Rotate resident 1-D device states of one shape and dtype in place, state
+    ``i`` by ``mats[i]`` (one matrix, or an ``(alpha, beta)`` pair whose ``None``
+    entries must be the same for every state), in one C-ABI call.  Small
+    states share their launches; each result is bitwise what
+    :func:`orbital_rotation_in_place` gives."""
+    states, mats = list(states), list(mats)
+    if len(mats) != len(states):
+        raise ValueError("one rotation per state")
+    if not states:
+        return states
+    na, nb = nelec
+    pairs = [_split_mat(m, norb) for m in mats]
+    has_a, has_b = pairs[0][0] is not None, pairs[0][1] is not None
+    tag = _device.dtype_tag(states[0])
+    for (a, b), t in zip(pairs, states):
+        if (a is not None) != has_a or (b is not None) != has_b:
+            raise ValueError("a batch needs the same sectors rotated in every state")
+        _device.check_size(t, norb, (na, nb))
+        if _device.dtype_tag(t) != tag or t.device != states[0].device:
+            raise ValueError("a batch needs states of one dtype on one device")
+    mats_a = np.ascontiguousarray(np.stack([a for a, _ in pairs])) if has_a else None
+    mats_b = np.ascontiguousarray(np.stack([b for _, b in pairs])) if has_b else None
+    # the states that need a work buffer run one after another: they share one
+    work = _device.work_buffer(states[0], norb, na, nb) if has_b else None
+    vec_ptrs = (ctypes.c_void_p * len(states))(*[t.data_ptr() for t in states])
+    work_ptrs = (ctypes.c_void_p * len(states))(
+        *[work.data_ptr() if work is not None else None for _ in states])
+    _lib.call("ffsim_apply_orbital_rotation_batched", len(states), vec_ptrs, work_ptrs, tag,
+              norb, na, nb, _device.host_ptr(mats_a), _device.host_ptr(mats_b),
+              _device.stream_ptr())
+    return states
 
 
 def apply_orbital_rotation(vec, mat, norb: int, nelec, *, copy: bool = True):

@@ -14,7 +14,7 @@ from ffsim_amd import linalg as _linalg
 from ffsim_amd._linop import DeviceLinearOperator
 from ffsim_amd.contract.diag_coulomb import contract_diag_coulomb_into
 from ffsim_amd.contract.num_op_sum import contract_num_op_sum_into
-from ffsim_amd.gates.orbital_rotation import orbital_rotation_in_place
+from ffsim_amd.gates.orbital_rotation import orbital_rotation_in_place, orbital_rotations_in_place
 
 
 def axpby_(out: torch.Tensor, x: torch.Tensor, y: torch.Tensor | None, alpha, beta=0.0):
@@ -35,6 +35,13 @@ def axpby_(out: torch.Tensor, x: torch.Tensor, y: torch.Tensor | None, alpha, be
         _device.stream_ptr(),
     )
     return out
+
+
+def _batch_terms(v: torch.Tensor, norb: int, nelec: tuple) -> bool:
+    """Whether the DF matvec rotates its terms' copies of v in shared launches:
+    for states without a transposition buffer (small ones), where one copy per
+    term costs little memory."""
+    return _device.work_elements(v, norb, *nelec) == 0
 
 
 @dataclasses.dataclass(frozen=True)
@@ -171,6 +178,20 @@ class DoubleFactorizedHamiltonian:
         def matvec_device(v: torch.Tensor) -> torch.Tensor:
             result = torch.empty_like(v)
             axpby_(result, v, None, constant)
+            if _batch_terms(v, norb, nelec):
+                # small states (no transposition buffer): one copy per term, and
+                # the terms' rotations share their launches (the same
+                # arithmetic per term as the loop below, in the same order)
+                ts = [v.clone() for _ in range(1 + len(mats))]
+                orbital_rotations_in_place(
+                    ts, [vecs.T.conj()] + [rot.T.conj() for rot in rots], norb, nelec)
+                contract_num_op_sum_into(ts[0], ts[0], eigs, norb, nelec)
+                for t, mat in zip(ts[1:], mats):
+                    contract_diag_coulomb_into(t, t, mat, norb, nelec, z_rep)
+                orbital_rotations_in_place(ts, [vecs] + rots, norb, nelec)
+                for t in ts:
+                    axpby_(result, result, t, 1.0, 1.0)
+                return result
             t = torch.empty_like(v)
             t.copy_(v)
             orbital_rotation_in_place(t, vecs.T.conj(), norb, nelec)

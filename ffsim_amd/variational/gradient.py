@@ -145,6 +145,13 @@ class DeviceAdjointOps:
 
         return apply_orbital_rotation(t, U, self.norb, self.nelec, copy=False)
 
+    def rotate_both(self, a, b, U):
+        """Both states rotated by U in one call (small states share launches)."""
+        from ffsim_amd.gates.orbital_rotation import orbital_rotations_in_place
+
+        orbital_rotations_in_place([a, b], [U, U], self.norb, self.nelec)
+        return a, b
+
     def diag(self, t, j_aa, j_ab, time):
         from ffsim_amd.gates.diag_coulomb import apply_diag_coulomb_evolution
 
@@ -184,7 +191,8 @@ def ucj_adjoint_sweep(ops, layout: UCJLayout, params: np.ndarray, psi_out, lam):
     """Backward sweep: (dL/dparams, dL/dconj(psi_0)) from psi_out and lam = dL/dconj(psi_out).
 
     ``ops`` supplies the primitive steps (rotate, diag, transition_rdm,
-    phase_reductions and their host reads).  ``psi_out`` and ``lam`` are not
+    phase_reductions and their host reads; optionally rotate_both, one call
+    that rotates psi and lam alike).  ``psi_out`` and ``lam`` are not
     modified.  The gate list is the merged form of _apply_unitary_
     (ucj_spin_balanced.py:657-696): V_1 = U_1^dag, D_1, V_2 = U_2^dag U_1, ...,
     D_R, V_{R+1} = F U_R.
@@ -210,8 +218,11 @@ def ucj_adjoint_sweep(ops, layout: UCJLayout, params: np.ndarray, psi_out, lam):
         # rotation V_{k+1} (0-based index k), measured after the gate
         rdms[k] = ops.transition_rdm(lam, psi)
         Vh = V[k].conj().T
-        psi = ops.rotate(psi, Vh)
-        lam = ops.rotate(lam, Vh)
+        if hasattr(ops, "rotate_both"):
+            psi, lam = ops.rotate_both(psi, lam, Vh)
+        else:
+            psi = ops.rotate(psi, Vh)
+            lam = ops.rotate(lam, Vh)
         if k > 0:
             j_aa, j_ab = J[k - 1]
             phases[k - 1] = ops.phase_reductions(psi, lam)

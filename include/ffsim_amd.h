@@ -90,6 +90,25 @@ int ffsim_apply_orbital_rotation(void* vec, void* work, int dtype, int norb, int
                                  int n_beta, const double* mat_a, const double* mat_b,
                                  void* stream);
 
+/* ffsim_apply_orbital_rotation for n_states states of one shape, each with its
+ * own matrices: batching has no counterpart in the reference, whose callers
+ * loop over apply_orbital_rotation (the factors of
+ * DoubleFactorizedHamiltonian._linear_operator_,
+ * python/ffsim/hamiltonians/double_factorized_hamiltonian.py:244-275).
+ * vecs / works: HOST arrays of n_states DEVICE pointers; works may be NULL
+ * (or hold NULLs) where ffsim_orbital_rotation_work_elements is 0, and holds
+ * buffers of that many elements otherwise (states that use one run one after
+ * another on the stream, so the entries may all name the same buffer).  mats_a / mats_b: NULL (that
+ * sector is not rotated) or n_states x norb x norb complex128.  Where a beta
+ * vector fits one tile (norb <= 12 in complex128, <= 14 in complex64) and
+ * norb >= 6, each window pass of each sector is one launch over all states;
+ * otherwise the states run one by one.  Each result is bitwise equal to
+ * ffsim_apply_orbital_rotation's. */
+int ffsim_apply_orbital_rotation_batched(int n_states, void* const* vecs, void* const* works,
+                                         int dtype, int norb, int n_alpha, int n_beta,
+                                         const double* mats_a, const double* mats_b,
+                                         void* stream);
+
 /* Replaces _lib.apply_diag_coulomb_evolution_in_place_num_rep / _z_rep
  * (src/gates/diag_coulomb.rs:19-190).  The matrices are real norb x norb,
  * already in the form ffsim's _get_mat_exp exponentiates

@@ -223,3 +223,27 @@ def test_givens_layer_argument_errors_without_gpu():
         call([0, 1], [1, 2])
     with pytest.raises(ValueError, match="out of range"):
         call([5], [6])
+
+
+def test_batched_rotation_argument_errors_without_gpu():
+    """The batched rotation checks its batch before touching the device."""
+    import torch
+
+    from ffsim_amd.gates.orbital_rotation import orbital_rotations_in_place
+
+    norb, nelec = 4, (2, 2)
+    states = [torch.zeros(36, dtype=torch.complex128) for _ in range(2)]
+    u = np.eye(norb)
+    with pytest.raises(ValueError, match="one rotation per state"):
+        orbital_rotations_in_place(states, [u], norb, nelec)
+    with pytest.raises(ValueError, match="same sectors"):
+        orbital_rotations_in_place(states, [(u, None), (u, u)], norb, nelec)
+    with pytest.raises(ValueError, match="one dtype"):
+        orbital_rotations_in_place([states[0], states[1].to(torch.complex64)], [u, u], norb,
+                                   nelec)
+    with pytest.raises(ValueError, match="entries"):
+        orbital_rotations_in_place([states[0], torch.zeros(35, dtype=torch.complex128)], [u, u],
+                                   norb, nelec)
+    with pytest.raises(ValueError, match="negative"):
+        _lib.check(_lib.lib().ffsim_apply_orbital_rotation_batched(
+            -1, None, None, _lib.C128, norb, 2, 2, None, None, None))

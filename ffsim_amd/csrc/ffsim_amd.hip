@@ -2423,7 +2423,7 @@ int ffsim_transition_rdm(const void* lam, const void* psi, int dtype, int norb, 
           hipLaunchKernelGGL(fill_zero_kernel,
                              dim3((unsigned)std::min<long>((nz + 255) / 256, 65536)), dim3(256), 0,
                              st, part, nz);
-          const dim3 grid((unsigned)((cnt + 3) / 4));
+          const dim3 grid((unsigned)cnt);  // one string per block, its links over 4 waves
           if (cols)
             hipLaunchKernelGGL((trdm_links_kernel<T, true>), grid, dim3(256), 0, st, l, x,
                                (int)n_str, (int)n_other, norb, strs, dtab, (int)s0, (int)cnt,

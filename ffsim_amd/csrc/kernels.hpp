@@ -1080,8 +1080,10 @@ __device__ __forceinline__ uint64_t colex_rank_dev(uint64_t s, const uint64_t* b
   return r;
 }
 
-// Transition one-body RDM of one spin sector, one wave per string S of the
-// sector: for q in S and p = q or p not in S, T = S - q + p and
+// Transition one-body RDM of one spin sector, one 4-wave block per string S
+// of the sector (its links dealt round-robin to the waves, so that a small
+// sector still puts several waves on every SIMD): for q in S and p = q or p
+// not in S, T = S - q + p and
 //   part[(p norb + q) 2 + c][S - s_begin] = sign * sum_o conj(lam(T, o)) psi(S, o)
 // (c: re, im) for the strings s_begin <= S < s_begin + n_chunk, where (S, o)
 // addresses row S of an n_str x n_other row-major matrix (COLS = false) or
@@ -1097,7 +1099,8 @@ __global__ void __launch_bounds__(256)
                       const uint64_t* __restrict__ binom_tab, int s_begin, int n_chunk,
                       double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
-  const int s_loc = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s_loc = blockIdx.x;
   const int s_idx = s_begin + s_loc;
   if (s_loc >= n_chunk || s_idx >= n_str) return;
   const uint64_t S = strings[s_idx];
@@ -1106,6 +1109,7 @@ __global__ void __launch_bounds__(256)
     return COLS ? m[(long)o * n_str + str] : m[str * n_other + o];
   };
   uint64_t qbits = S;
+  int link = 0;  // the string's links in (q, p) order, dealt round-robin to the 4 waves
   while (qbits) {
     const int q = __builtin_ctzll(qbits);
     qbits &= qbits - 1;
@@ -1113,6 +1117,7 @@ __global__ void __launch_bounds__(256)
     const int sign_q = __popcll(S & ((1ull << q) - 1)) & 1;
     for (int p = 0; p < norb; ++p) {
       if (p != q && ((S >> p) & 1)) continue;
+      if ((link++ & 3) != wave) continue;
       const uint64_t S1 = S0 | (1ull << p);
       const int sign_p = __popcll(S0 & ((1ull << p) - 1)) & 1;
       const double sgn = ((sign_q ^ sign_p) & 1) ? -1.0 : 1.0;

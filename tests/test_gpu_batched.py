@@ -38,7 +38,8 @@ def _pairs(norb):
 @pytest.mark.parametrize(
     "norb,nelec,n_states,final",
     [(4, (2, 1), 3, False), (6, (3, 3), 5, True), (8, (4, 4), 7, False), (8, (4, 3), 4, True),
-     (10, (5, 5), 6, False), (10, (4, 5), 3, True), (12, (6, 6), 3, False), (13, (6, 5), 2, True)],
+     (10, (5, 5), 6, False), (10, (4, 5), 3, True), (12, (6, 6), 3, False), (13, (6, 5), 2, True),
+     (9, (4, 0), 3, False)],
 )
 def test_batched_ucj_matches_oracle(norb, nelec, n_states, final, dtype, path, monkeypatch):
     """path: the library's choice, or FFSIM_AMD_BATCH_PATH forcing the GEMM path
@@ -137,7 +138,8 @@ def test_batched_multi_state_mixed_structures(dtype, monkeypatch):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("norb,nelec", [(5, (2, 2)), (8, (4, 4)), (12, (6, 5)), (13, (6, 6))])
+@pytest.mark.parametrize("norb,nelec", [(5, (2, 2)), (8, (4, 4)), (12, (6, 5)), (13, (6, 6)),
+                                        (8, (3, 0)), (7, (0, 2))])
 @pytest.mark.parametrize("form", ["one", "alpha", "beta", "pair"])
 def test_batched_orbital_rotation_bitwise(norb, nelec, form, dtype):
     """ffsim_apply_orbital_rotation_batched, each state with its own rotation:

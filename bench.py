@@ -385,25 +385,32 @@ def _native_calls(vec, norb: int, nocc: int, rng, reps: int) -> dict:
 
 
 def _one_body(psi, norb: int, nelec, rng, reps: int) -> dict:
-    """contract_one_body (both sectors, out = M^ psi) with a random complex M on
-    the resident state: wall ms per call and the state passes' rate (psi read
-    twice, out written, then read and written: 5 state sizes per call)."""
+    """contract_one_body (both sectors, out = M^ psi) on the resident state with
+    a random complex M (the window passes; their rate counts psi read twice,
+    out written, then read and written: 5 state sizes per call) and with a
+    random Hermitian M (two orbital rotations around a number-operator pass):
+    wall ms per call."""
     import torch
 
     from ffsim_amd.contract.one_body import contract_one_body_into
 
-    mat = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
-    out = torch.empty_like(psi)
-    contract_one_body_into(psi, out, mat, norb, nelec)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    def timed(mat):
+        out = torch.empty_like(psi)
         contract_one_body_into(psi, out, mat, norb, nelec)
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / reps
-    del out
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            contract_one_body_into(psi, out, mat, norb, nelec)
+        torch.cuda.synchronize()
+        del out
+        return (time.perf_counter() - t0) / reps
+
+    mat = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    t = timed(mat)
+    t_h = timed(mat + mat.T.conj())
     return {"ms_per_call": round(t * 1e3, 3),
-            "state_pass_GBs": round(5 * psi.numel() * psi.element_size() / t / 1e9, 1)}
+            "state_pass_GBs": round(5 * psi.numel() * psi.element_size() / t / 1e9, 1),
+            "hermitian_ms_per_call": round(t_h * 1e3, 3)}
 
 
 def numpy_io_leg(op, norb: int, nelec, reps: int = 3, seed: int = 0) -> dict:

@@ -98,11 +98,11 @@ int ffsim_apply_orbital_rotation(void* vec, void* work, int dtype, int norb, int
  * vecs / works: HOST arrays of n_states DEVICE pointers; works may be NULL
  * (or hold NULLs) where ffsim_orbital_rotation_work_elements is 0, and holds
  * buffers of that many elements otherwise (states that use one run one after
- * another on the stream, so the entries may all name the same buffer).  mats_a / mats_b: NULL (that
- * sector is not rotated) or n_states x norb x norb complex128.  Where a beta
- * vector fits one tile (norb <= 12 in complex128, <= 14 in complex64) and
- * norb >= 6, each window pass of each sector is one launch over all states;
- * otherwise the states run one by one.  Each result is bitwise equal to
+ * another on the stream, so the entries may all name the same buffer).
+ * mats_a / mats_b: NULL (that sector is not rotated) or n_states x norb x
+ * norb complex128.  Where a beta vector fits one tile (6 <= norb <= 12),
+ * each window pass of each sector is one launch over all states; otherwise
+ * the states run one by one.  Each result is bitwise equal to
  * ffsim_apply_orbital_rotation's. */
 int ffsim_apply_orbital_rotation_batched(int n_states, void* const* vecs, void* const* works,
                                          int dtype, int norb, int n_alpha, int n_beta,
@@ -174,9 +174,9 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
  * compound matrix Lambda^k(U) (built on the device) applied to all states by
  * batched complex GEMMs, and each diagonal term one batched elementwise pass.
  * Other states take the Givens path.  Where a beta vector fits one tile
- * (norb <= 12 in complex128, <= 14 in complex64) and norb >= 6, every window
- * pass and diagonal term is one launch over all states, with results bitwise
- * equal to the single-state entry's; otherwise the states run one by one on
+ * (6 <= norb <= 12), every window pass and diagonal term is one launch over
+ * all states, with results bitwise equal to the single-state entry's;
+ * otherwise the states run one by one on
  * the stream (on the default stream, small states run on up to 4
  * library-owned streams at once, between events on the default stream). */
 int ffsim_apply_ucj_spin_balanced_batched(int n_states, void* const* vecs, void* const* works,

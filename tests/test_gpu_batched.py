@@ -144,10 +144,9 @@ def test_batched_multi_state_mixed_structures(dtype, monkeypatch):
 def test_batched_orbital_rotation_bitwise(norb, nelec, form, dtype):
     """ffsim_apply_orbital_rotation_batched, each state with its own rotation:
     one matrix for both sectors, alpha only, beta only, or an (alpha, beta)
-    pair.  norb 8 and 12 share launches across the states (as does norb 13 in
-    complex64); norb 5 (windows narrower than a register sub-window) and
-    norb 13 in complex128 (beta sector transposed through one shared work
-    buffer) run state by state.  Every result equals the single-state entry's
+    pair.  norb 7, 8 and 12 share launches across the states; norb 5
+    (windows narrower than a register sub-window) and norb 13 (beta sector
+    transposed through one shared work buffer) run state by state.  Every result equals the single-state entry's
     bit for bit and the oracle's (complex128 1e-12, complex64 2e-6)."""
     import torch
 

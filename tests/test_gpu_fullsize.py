@@ -281,24 +281,29 @@ def test_norb18_c64_trotter_step_error_vs_c128_and_round_trip():
 
 
 @pytest.mark.parametrize(
-    "norb,nelec,dtype", [(16, (8, 8), "c128"), (16, (8, 8), "c64"), (14, (7, 6), "c128")]
+    "norb,nelec,dtype,hermitian",
+    [(16, (8, 8), "c128", False), (16, (8, 8), "c64", False), (14, (7, 6), "c128", False),
+     (16, (8, 8), "c128", True), (16, (8, 8), "c64", True)],
 )
-def test_product_state_one_body_contraction(norb, nelec, dtype):
+def test_product_state_one_body_contraction(norb, nelec, dtype, hermitian):
     """contract_one_body with a random complex M on psi = a (x) b gives
     (M a) (x) b + a (x) (M b), each factor from the oracle's single-sector
     contraction (pyscf's contract_1e restated).  Above 12 orbitals the
     contraction runs as three window passes per sector, one of them over a
-    split window."""
+    split window; a Hermitian M runs as two orbital rotations around a
+    number-operator pass."""
     from ffsim_amd.contract.one_body import contract_one_body_into
 
     torch = _torch()
     rng = np.random.default_rng(norb * 10 + (dtype == "c64"))
     a, b = _factors(norb, nelec, rng)
     M = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    if hermitian:
+        M = M + M.T.conj()
     ma = O.contract_one_body(a, M, norb, (nelec[0], 0))
     mb = O.contract_one_body(b, M, norb, (nelec[1], 0))
     da, db = len(a), len(b)
-    _case("product_state_one_body", norb, nelec, dtype, da, db)
+    _case("product_state_one_body", norb, nelec, dtype, hermitian, da, db)
     t = _product_state(a, b, dtype)
     out = torch.empty_like(t)
     contract_one_body_into(t, out, M, norb, nelec)

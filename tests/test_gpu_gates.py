@@ -487,3 +487,35 @@ def test_ucj_fused_diagonal_with_trivial_orbital_phases(rng):
                                          norb, nelec)
     got = ffsim_amd.apply_unitary(vec, op, norb, nelec)
     np.testing.assert_allclose(got, expected, atol=TOL, rtol=0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_contract_one_body_hermitian_rotated_path(dtype, accumulate, rng):
+    """A Hermitian M on a state that needs the transposition buffer (norb=13
+    complex128, norb=15 complex64) runs as rotate / number operators / rotate
+    back; against the oracle's link-table contraction, with scale and
+    accumulation (complex128 1e-12; complex64 relative 2e-6)."""
+    import torch
+
+    from ffsim_amd.contract.one_body import _eigen_path, contract_one_body_into
+
+    norb, nelec = (13, (6, 5)) if dtype == "c128" else (15, (4, 3))
+    _case("one-body hermitian path", norb, nelec, dtype, accumulate)
+    vec = _rand(norb, nelec, rng)
+    a = rng.standard_normal((norb, norb)) + 1j * rng.standard_normal((norb, norb))
+    mat = a + a.T.conj()
+    cdt = torch.complex128 if dtype == "c128" else torch.complex64
+    src = torch.from_numpy(vec).to("cuda").to(cdt)
+    assert _eigen_path(np.ascontiguousarray(mat), src, norb, nelec)
+    prev = _rand(norb, nelec, rng)
+    out = torch.from_numpy(prev).to("cuda").to(cdt)
+    contract_one_body_into(src, out, mat, norb, nelec, scale=0.5, accumulate=accumulate)
+    expected = 0.5 * O.contract_one_body(vec, mat, norb, nelec) + (prev if accumulate else 0)
+    got = out.cpu().numpy().astype(np.complex128)
+    if dtype == "c128":
+        np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)
+    else:
+        assert np.abs(got - expected).max() <= 2e-6 * np.abs(expected).max()
+    # a matrix that is not exactly Hermitian takes the window passes
+    assert not _eigen_path(np.ascontiguousarray(mat + 1e-13j * np.eye(norb)), src, norb, nelec)

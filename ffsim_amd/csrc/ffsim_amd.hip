@@ -668,21 +668,28 @@ size_t fiber_lds_bytes(size_t elem, int W, bool cols, int n_waves, int rows, int
          align16((size_t)gd_cap * 4) + diag_lds_bytes(diag, fuse, n_diag, rows, W, n_fac, win);
 }
 
-template <typename T, int W, int MODE, int W2, int NTHR, bool FUSE = false>
+// mt: several states of one shape (in place: src and dst unused; no
+// transposing store, no fused diagonal), their tiles in one persistent
+// launch; null for one state.
+template <typename T, int W, int MODE, int W2, int NTHR, bool FUSE = false, bool MULTI = false>
 void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_batch, bool tstore,
                     const DevWindow& win, bool all_fibers, const DevGroups& gr, const LoSlots& los,
                     int gd_cap, int gm_cap, const SubPass* d_subs, int n_sub,
                     const LayerPair* d_layers, int n_layers, const double* d_diag,
                     const uint64_t* d_strings, int n_diag, size_t lds_max, hipStream_t stream,
-                    const DiagFuse& fz = DiagFuse{}) {
+                    const DiagFuse& fz = DiagFuse{}, const MultiTiles* mt = nullptr) {
+  static_assert(!(MULTI && FUSE), "the multi-state pass has no fused diagonal");
+  require(MULTI == (mt != nullptr) && (!MULTI || !tstore), FFSIM_ERR_VALUE,
+          "fiber launch: inconsistent multi-state arguments");
+  const int n_states = MULTI ? mt->n_states : 1;
   const int n_f = all_fibers ? win.n_fibers : win.n_active;
-  if (n_f == 0 || n_batch == 0) return;
+  if (n_f == 0 || n_batch == 0 || n_states == 0) return;
   const int rows = all_fibers ? win.max_rows : win.max_rows_active;
   const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, MODE == MODE_COLS, NTHR / 64, rows,
                                      gd_cap, gm_cap, d_diag != nullptr, FUSE, n_diag, fz.n_fac,
                                      fz.win);
   require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
-  auto kern = fiber_kernel<T, W, MODE, NTHR, W2, FUSE>;
+  auto kern = fiber_kernel<T, W, MODE, NTHR, W2, FUSE, MULTI>;
   const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
   static bool attr_set = false;
   if (!attr_set) {
@@ -691,7 +698,7 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
     attr_set = true;
   }
   const int n_bt = (n_batch + W - 1) / W;
-  const long n_tiles = (long)n_bt * n_f;
+  const long n_tiles = (long)n_bt * n_f * n_states;
   require(n_tiles < (1l << 40), FFSIM_ERR_VALUE, "grid too large");
   // persistent grid: as many blocks as fit on the device at once
   int occ = 0;
@@ -699,54 +706,14 @@ void launch_fiber_t(const void* src, void* dst, long ld_src, long ld_dst, int n_
   const long per_cu = std::max(1, occ);
   const long grid = std::min<long>(n_tiles, per_cu * g_n_cu);
   // algorithmic bytes: every element of every launched fiber read once and written once
-  const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch *
+  const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch * (double)n_states *
                        (double)(all_fibers ? win.rows_all : win.rows_active);
   const long pk = prof_start(PROF_FIBER, moved, stream);
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
                      reinterpret_cast<const Cx<T>*>(src), reinterpret_cast<Cx<T>*>(dst), ld_src,
                      ld_dst, n_batch, tstore ? 1 : 0, win.fibers, n_f, win.rows, gr.args(sched), los,
                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap,
-                     gm_cap, fz, MultiTiles{nullptr, 1, 0, 0});
-  HIP_CHECK(hipGetLastError());
-  prof_stop(pk, stream);
-}
-
-// One window pass over several states of one shape (in place, no transposing
-// store, no fused diagonal): the states' tiles in one persistent launch.
-template <typename T, int W, int MODE, int NTHR>
-void launch_fiber_multi_t(const MultiTiles& mt, long ld, int n_batch, const DevWindow& win,
-                          bool all_fibers, const DevGroups& gr, const LoSlots& los, int gd_cap,
-                          int gm_cap, const SubPass* d_subs, int n_sub, const LayerPair* d_layers,
-                          int n_layers, const double* d_diag, const uint64_t* d_strings,
-                          int n_diag, size_t lds_max, hipStream_t stream) {
-  const int n_f = all_fibers ? win.n_fibers : win.n_active;
-  if (n_f == 0 || n_batch == 0 || mt.n_states == 0) return;
-  const int rows = all_fibers ? win.max_rows : win.max_rows_active;
-  const size_t lds = fiber_lds_bytes(sizeof(Cx<T>), W, MODE == MODE_COLS, NTHR / 64, rows,
-                                     gd_cap, gm_cap, d_diag != nullptr, false, n_diag, 0, 0);
-  require(lds <= lds_max, FFSIM_ERR_VALUE, "fiber tile exceeds LDS");
-  auto kern = fiber_kernel<T, W, MODE, NTHR, kMaxSubWindow, false, true>;
-  const DevSchedule& sched = dev_schedule(context(), gr, NTHR / 64, W);
-  static bool attr_set = false;
-  if (!attr_set) {
-    HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_max));
-    attr_set = true;
-  }
-  const int n_bt = (n_batch + W - 1) / W;
-  const long n_tiles = (long)n_bt * n_f * mt.n_states;
-  require(n_tiles < (1l << 40), FFSIM_ERR_VALUE, "grid too large");
-  int occ = 0;
-  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, NTHR, lds));
-  const long grid = std::min<long>(n_tiles, (long)std::max(1, occ) * g_n_cu);
-  const double moved = 2.0 * (double)sizeof(Cx<T>) * (double)n_batch * (double)mt.n_states *
-                       (double)(all_fibers ? win.rows_all : win.rows_active);
-  const long pk = prof_start(PROF_FIBER, moved, stream);
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTHR), lds, stream,
-                     static_cast<const Cx<T>*>(nullptr), static_cast<Cx<T>*>(nullptr), ld, ld,
-                     n_batch, 0, win.fibers, n_f, win.rows, gr.args(sched), los, d_subs, n_sub,
-                     d_layers, n_layers, d_diag, d_strings, n_diag, rows, gd_cap, gm_cap,
-                     DiagFuse{}, mt);
+                     gm_cap, fz, MULTI ? *mt : MultiTiles{nullptr, 1, 0, 0});
   HIP_CHECK(hipGetLastError());
   prof_stop(pk, stream);
 }
@@ -766,9 +733,10 @@ void launch_fiber_multi(int mode, const MultiTiles& mt, long ld, int n_batch,
                                        gm_cap, d_diag != nullptr, false, n_diag, 0, 0);
   const bool wide = 2 * lds_l > lds_max;
 #define FFSIM_MULTI_LAUNCH(MM, NT)                                                           \
-  launch_fiber_multi_t<T, WW, MM, NT>(mt, ld, n_batch, win, all_fibers, gr, los, gd_cap, gm_cap, \
-                                      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings,   \
-                                      n_diag, lds_max, stream)
+  launch_fiber_t<T, WW, MM, kMaxSubWindow, NT, false, true>(                                 \
+      nullptr, nullptr, ld, ld, n_batch, false, win, all_fibers, gr, los, gd_cap, gm_cap,      \
+      d_subs, n_sub, d_layers, n_layers, d_diag, d_strings, n_diag, lds_max, stream,           \
+      DiagFuse{}, &mt)
   if (mode == MODE_ROWS) {
     if (wide) FFSIM_MULTI_LAUNCH(MODE_ROWS, 512); else FFSIM_MULTI_LAUNCH(MODE_ROWS, 256);
   } else {

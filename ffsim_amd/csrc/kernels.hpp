@@ -1115,6 +1115,16 @@ __global__ void __launch_bounds__(256)
   auto at = [&](const Cx<T>* m, long str, int o) -> Cx<T> {
     return COLS ? m[(long)o * n_str + str] : m[str * n_other + o];
   };
+  // psi's row in registers when it is at most 16 values per lane (row-major
+  // sectors): each link then reads only lam's row
+  constexpr int XR = 16;
+  const bool xreg = !COLS && n_other <= 64 * XR;
+  Cx<T> xr[XR];
+#pragma unroll
+  for (int k = 0; k < XR; ++k) {
+    const int o = lane + 64 * k;
+    xr[k] = (xreg && o < n_other) ? psi[(long)s_idx * n_other + o] : Cx<T>{T(0), T(0)};
+  }
   uint64_t qbits = S;
   int link = 0;  // the string's links in (q, p) order, dealt round-robin to the 4 waves
   while (qbits) {
@@ -1130,12 +1140,25 @@ __global__ void __launch_bounds__(256)
       const double sgn = ((sign_q ^ sign_p) & 1) ? -1.0 : 1.0;
       const long t_idx = (long)colex_rank_dev(S1, binom_tab, stride);
       double re = 0.0, im = 0.0;
+      if (xreg) {
+        // the same terms in the same order as the loop below
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+          const int o = lane + 64 * k;
+          if (o < n_other) {
+            const Cx<T> l = lam[t_idx * n_other + o], x = xr[k];
+            re += (double)l.re * (double)x.re + (double)l.im * (double)x.im;
+            im += (double)l.re * (double)x.im - (double)l.im * (double)x.re;
+          }
+        }
+      } else {
 #pragma unroll 4
-      for (int o = lane; o < n_other; o += 64) {
-        const Cx<T> l = at(lam, t_idx, o), x = at(psi, s_idx, o);
-        // conj(l) * x
-        re += (double)l.re * (double)x.re + (double)l.im * (double)x.im;
-        im += (double)l.re * (double)x.im - (double)l.im * (double)x.re;
+        for (int o = lane; o < n_other; o += 64) {
+          const Cx<T> l = at(lam, t_idx, o), x = at(psi, s_idx, o);
+          // conj(l) * x
+          re += (double)l.re * (double)x.re + (double)l.im * (double)x.im;
+          im += (double)l.re * (double)x.im - (double)l.im * (double)x.re;
+        }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {

@@ -36,6 +36,8 @@ SIGNATURES: dict[str, list] = {
     "ffsim_apply_orbital_rotation": [_P, _P, _I, _I, _I, _I, _P, _P, _P],
     "ffsim_apply_orbital_rotation_batched": [_I, _P, _P, _I, _I, _I, _I, _P, _P, _P],
     "ffsim_apply_diag_coulomb_evolution": [_P, _I, _I, _I, _I, _P, _P, _P, _D, _I, _P],
+    "ffsim_apply_diag_coulomb_evolution_batched": [_I, _P, _I, _I, _I, _I, _P, _P, _P, _D, _I,
+                                                   _P],
     "ffsim_apply_num_op_sum_evolution": [_P, _I, _I, _I, _I, _P, _P, _D, _P],
     "ffsim_contract_diag_coulomb": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _D, _I, _P],
     "ffsim_contract_num_op_sum": [_P, _P, _I, _I, _I, _I, _P, _P, _D, _I, _P],

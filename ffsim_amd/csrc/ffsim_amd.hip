@@ -1172,6 +1172,53 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   prog.finish();
 }
 
+// One diagonal evolution applied to several states of one shape in place: the
+// column factors once, then one elementwise launch with grid.z over the
+// states, each state through the same arithmetic as diag_impl's.
+void diag_evolve_batched_impl(Context& ctx, int n_states, void* const* vecs, int dtype, int norb,
+                              int nrow_occ, int ncol_occ, const DiagSpec& spec,
+                              hipStream_t stream) {
+  const long n_rows = (long)binom(norb, nrow_occ), n_cols = (long)binom(norb, ncol_occ);
+  if (n_rows == 0 || n_cols == 0 || n_states == 0) return;
+  require(norb <= 63, FFSIM_ERR_VALUE, "norb > 63 unsupported");
+  require(spec.norb == norb, FFSIM_ERR_VALUE, "diagonal operator of another norb");
+  require(diag_grid(n_rows, n_cols).y <= 65535, FFSIM_ERR_VALUE, "too many columns");
+  const DevStrings& rs = dev_strings(ctx, norb, nrow_occ);
+  const DevStrings& cs = dev_strings(ctx, norb, ncol_occ);
+  constexpr int kChunk = 256;
+  for (int i0 = 0; i0 < n_states; i0 += kChunk) {
+    const int m = std::min(kChunk, n_states - i0);
+    Program prog(ctx, stream);
+    // one header per state (the kernel indexes them by state), one coefficient set
+    const std::vector<DiagHeader> hs(m, diag_header(spec));
+    const size_t hoff = prog.append(hs.data(), hs.size() * sizeof(DiagHeader));
+    const std::vector<double> coef = diag_coefficients(spec);
+    const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
+    std::vector<const void*> ptrs(vecs + i0, vecs + i0 + m);
+    const size_t soff = prog.append(ptrs.data(), m * sizeof(void*));
+    size_t goff = 0;
+    require(prog.reserve((size_t)n_cols * 2 * sizeof(double), &goff), FFSIM_ERR_VALUE,
+            "diagonal operator: too many columns");
+    prog.commit();
+    const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
+    const double* dc = prog.dev<double>(coff);
+    double* dg = prog.scratch<double>(goff);
+    launch_diag_col_term(n_cols, cs.d, dh, dc, dg, 1, stream);
+    HIP_CHECK(hipGetLastError());
+    const double moved = 2.0 * (double)elem_bytes(dtype) * (double)n_rows * (double)n_cols * m;
+    const long pk = prof_start(PROF_DIAG, moved, stream);
+    if (dtype == FFSIM_C128)
+      launch_diag_pass_t<double>(DIAG_EVOLVE, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg,
+                                 dh, dc, stream, prog.dev<void*>(soff), m, 0, 0);
+    else
+      launch_diag_pass_t<float>(DIAG_EVOLVE, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg,
+                                dh, dc, stream, prog.dev<void*>(soff), m, 0, 0);
+    HIP_CHECK(hipGetLastError());
+    prof_stop(pk, stream);
+    prog.finish();
+  }
+}
+
 std::vector<double> vec_or_empty(const double* m, int n) {
   if (!m) return {};
   return std::vector<double>(m, m + n);
@@ -2077,6 +2124,27 @@ int ffsim_apply_diag_coulomb_evolution(void* vec, int dtype, int norb, int n_alp
     DiagSpec s = diag_coulomb_spec(norb, mat_aa, mat_ab, mat_bb, z_representation, time);
     if (!s.flags) return;
     diag_impl(ctx, DIAG_EVOLVE, vec, vec, dtype, norb, n_alpha, n_beta, s, (hipStream_t)stream);
+  });
+}
+
+int ffsim_apply_diag_coulomb_evolution_batched(int n_states, void* const* vecs, int dtype,
+                                               int norb, int n_alpha, int n_beta,
+                                               const double* mat_aa, const double* mat_ab,
+                                               const double* mat_bb, double time,
+                                               int z_representation, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(n_states >= 0, FFSIM_ERR_VALUE, "negative batch size");
+    if (n_states == 0) return;
+    require(vecs != nullptr, FFSIM_ERR_VALUE, "batched diagonal evolution: no state array");
+    for (int i = 0; i < n_states; ++i)
+      require(vecs[i] != nullptr, FFSIM_ERR_VALUE, "batched diagonal evolution: null state");
+    Context& ctx = context();
+    DiagSpec s = diag_coulomb_spec(norb, mat_aa, mat_ab, mat_bb, z_representation, time);
+    if (!s.flags) return;
+    diag_evolve_batched_impl(ctx, n_states, vecs, dtype, norb, n_alpha, n_beta, s,
+                             (hipStream_t)stream);
   });
 }
 

@@ -8,6 +8,8 @@ computes E(a, b) per determinant and multiplies by exp(-i t E) in one pass.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -37,6 +39,41 @@ def prepared_mats(mat, norb: int):
     bb = _halved(mat_bb, norb) if mat_bb is not None else None
     ab = _device.host_mat(mat_ab, (norb, norb), dtype=float) if mat_ab is not None else None
     return aa, ab, bb
+
+
+def diag_coulomb_evolutions_in_place(
+    states, mat, time: float, norb: int, nelec, z_representation: bool
+) -> list:
+    """One diagonal Coulomb evolution on several resident states of one shape and
+    dtype, in one C-ABI call (one elementwise launch over all of them); each
+    result is bitwise what :func:`diag_coulomb_evolution_in_place` gives."""
+    states = list(states)
+    if not states:
+        return states
+    aa, ab, bb = prepared_mats(mat, norb)
+    na, nb = nelec
+    tag = _device.dtype_tag(states[0])
+    for t in states:
+        _device.check_size(t, norb, nelec)
+        if _device.dtype_tag(t) != tag or t.device != states[0].device:
+            raise ValueError("a batch needs states of one dtype on one device")
+    ptrs = (ctypes.c_void_p * len(states))(*[t.data_ptr() for t in states])
+    _lib.call(
+        "ffsim_apply_diag_coulomb_evolution_batched",
+        len(states),
+        ptrs,
+        tag,
+        norb,
+        na,
+        nb,
+        _device.host_ptr(aa),
+        _device.host_ptr(ab),
+        _device.host_ptr(bb),
+        float(time),
+        1 if z_representation else 0,
+        _device.stream_ptr(),
+    )
+    return states
 
 
 def diag_coulomb_evolution_in_place(

@@ -159,6 +159,14 @@ class DeviceAdjointOps:
             t, (j_aa, j_ab, j_aa), time, self.norb, self.nelec, copy=False
         )
 
+    def diag_both(self, a, b, j_aa, j_ab, time):
+        """Both states evolved by the same diagonal gate in one launch."""
+        from ffsim_amd.gates.diag_coulomb import diag_coulomb_evolutions_in_place
+
+        diag_coulomb_evolutions_in_place([a, b], (j_aa, j_ab, j_aa), time, self.norb,
+                                         self.nelec, False)
+        return a, b
+
     def transition_rdm(self, lam, psi):
         out = torch.empty((2, self.norb, self.norb), dtype=torch.complex128, device=psi.device)
         na, nb = self.nelec
@@ -191,8 +199,8 @@ def ucj_adjoint_sweep(ops, layout: UCJLayout, params: np.ndarray, psi_out, lam):
     """Backward sweep: (dL/dparams, dL/dconj(psi_0)) from psi_out and lam = dL/dconj(psi_out).
 
     ``ops`` supplies the primitive steps (rotate, diag, transition_rdm,
-    phase_reductions and their host reads; optionally rotate_both, one call
-    that rotates psi and lam alike).  ``psi_out`` and ``lam`` are not
+    phase_reductions and their host reads; optionally rotate_both and
+    diag_both, one call that transforms psi and lam alike).  ``psi_out`` and ``lam`` are not
     modified.  The gate list is the merged form of _apply_unitary_
     (ucj_spin_balanced.py:657-696): V_1 = U_1^dag, D_1, V_2 = U_2^dag U_1, ...,
     D_R, V_{R+1} = F U_R.
@@ -226,8 +234,11 @@ def ucj_adjoint_sweep(ops, layout: UCJLayout, params: np.ndarray, psi_out, lam):
         if k > 0:
             j_aa, j_ab = J[k - 1]
             phases[k - 1] = ops.phase_reductions(psi, lam)
-            psi = ops.diag(psi, j_aa, j_ab, 1.0)
-            lam = ops.diag(lam, j_aa, j_ab, 1.0)
+            if hasattr(ops, "diag_both"):
+                psi, lam = ops.diag_both(psi, lam, j_aa, j_ab, 1.0)
+            else:
+                psi = ops.diag(psi, j_aa, j_ab, 1.0)
+                lam = ops.diag(lam, j_aa, j_ab, 1.0)
 
     rho = [ops.host_rdm(x).sum(axis=0) for x in rdms]
     grad = np.zeros(layout.n_params)

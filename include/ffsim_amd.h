@@ -121,6 +121,17 @@ int ffsim_apply_diag_coulomb_evolution(void* vec, int dtype, int norb, int n_alp
                                        const double* mat_bb, double time, int z_representation,
                                        void* stream);
 
+/* ffsim_apply_diag_coulomb_evolution with one operator on n_states states of
+ * one shape (vecs: a HOST array of DEVICE pointers), in one elementwise
+ * launch over all of them; batching has no counterpart in the reference,
+ * whose callers loop (the adjoint sweep here evolves psi and lambda alike).
+ * Each result is bitwise equal to ffsim_apply_diag_coulomb_evolution's. */
+int ffsim_apply_diag_coulomb_evolution_batched(int n_states, void* const* vecs, int dtype,
+                                               int norb, int n_alpha, int n_beta,
+                                               const double* mat_aa, const double* mat_ab,
+                                               const double* mat_bb, double time,
+                                               int z_representation, void* stream);
+
 /* Replaces _lib.apply_num_op_sum_evolution_in_place on both spin sectors
  * (src/gates/num_op_sum.rs:18-36, python/ffsim/gates/num_op_sum.py:175-219).
  * coeffs_a / coeffs_b: norb real coefficients or NULL. */

@@ -247,3 +247,20 @@ def test_batched_rotation_argument_errors_without_gpu():
     with pytest.raises(ValueError, match="negative"):
         _lib.check(_lib.lib().ffsim_apply_orbital_rotation_batched(
             -1, None, None, _lib.C128, norb, 2, 2, None, None, None))
+
+
+def test_batched_diag_argument_errors_without_gpu():
+    """The batched diagonal evolution checks its batch before touching the device."""
+    import torch
+
+    from ffsim_amd.gates.diag_coulomb import diag_coulomb_evolutions_in_place
+
+    norb, nelec = 4, (2, 2)
+    j = np.eye(norb)
+    with pytest.raises(ValueError, match="one dtype"):
+        diag_coulomb_evolutions_in_place(
+            [torch.zeros(36, dtype=torch.complex128), torch.zeros(36, dtype=torch.complex64)],
+            j, 1.0, norb, nelec, False)
+    with pytest.raises(ValueError, match="negative"):
+        _lib.check(_lib.lib().ffsim_apply_diag_coulomb_evolution_batched(
+            -1, None, _lib.C128, norb, 2, 2, None, None, None, 1.0, 0, None))

@@ -40,6 +40,7 @@ SIGNATURES: dict[str, list] = {
                                                    _P],
     "ffsim_apply_num_op_sum_evolution": [_P, _I, _I, _I, _I, _P, _P, _D, _P],
     "ffsim_contract_diag_coulomb": [_P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _D, _I, _P],
+    "ffsim_contract_diag_coulomb_batched": [_I, _P, _I, _I, _I, _I, _P, _P, _P, _I, _D, _P],
     "ffsim_contract_num_op_sum": [_P, _P, _I, _I, _I, _I, _P, _P, _D, _I, _P],
     "ffsim_axpby": [_P, _P, _P, _I, _I64, _D, _D, _D, _D, _P],
     "ffsim_apply_ucj_spin_balanced": [_P, _P, _I, _I, _I, _I, _I, _P, _P, _P, _P],

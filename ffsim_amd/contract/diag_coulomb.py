@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -66,6 +68,51 @@ def contract_diag_coulomb_into(
         _device.stream_ptr(),
     )
     return out
+
+
+def contract_diag_coulombs_in_place(states, mats, norb: int, nelec,
+                                    z_representation: bool = False, scale: float = 1.0) -> list:
+    """state_i = scale * E_i(a, b) * state_i for resident states of one shape and
+    dtype, each with its own diagonal Coulomb operator (``mats[i]`` in any form
+    :func:`contract_diag_coulomb_into` takes; the same entries None in all), in
+    one C-ABI call and one elementwise launch; each result is bitwise what
+    :func:`contract_diag_coulomb_into` gives with ``out`` the state itself."""
+    states, mats = list(states), list(mats)
+    if len(mats) != len(states):
+        raise ValueError("one operator per state")
+    if not states:
+        return states
+    prepared = [get_mats(m, norb, z_representation) for m in mats]
+    tag = _device.dtype_tag(states[0])
+    for t in states:
+        _device.check_size(t, norb, nelec)
+        if _device.dtype_tag(t) != tag or t.device != states[0].device:
+            raise ValueError("a batch needs states of one dtype on one device")
+    stacked = []
+    for k in range(3):
+        present = [p[k] is not None for p in prepared]
+        if any(present) and not all(present):
+            raise ValueError("a batch needs the same interactions in every operator")
+        stacked.append(np.ascontiguousarray(np.stack([p[k] for p in prepared]))
+                       if present[0] else None)
+    na, nb = nelec
+    ptrs = (ctypes.c_void_p * len(states))(*[t.data_ptr() for t in states])
+    _lib.call(
+        "ffsim_contract_diag_coulomb_batched",
+        len(states),
+        ptrs,
+        tag,
+        norb,
+        na,
+        nb,
+        _device.host_ptr(stacked[0]),
+        _device.host_ptr(stacked[1]),
+        _device.host_ptr(stacked[2]),
+        1 if z_representation else 0,
+        float(scale),
+        _device.stream_ptr(),
+    )
+    return states
 
 
 def contract_diag_coulomb(vec, mat, norb: int, nelec, *, z_representation: bool = False):

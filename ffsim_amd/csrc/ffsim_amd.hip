@@ -1172,47 +1172,68 @@ void diag_impl(Context& ctx, int op, const void* src, void* dst, int dtype, int 
   prog.finish();
 }
 
-// One diagonal evolution applied to several states of one shape in place: the
-// column factors once, then one elementwise launch with grid.z over the
-// states, each state through the same arithmetic as diag_impl's.
-void diag_evolve_batched_impl(Context& ctx, int n_states, void* const* vecs, int dtype, int norb,
-                              int nrow_occ, int ncol_occ, const DiagSpec& spec,
-                              hipStream_t stream) {
+// Diagonal operators applied to several states of one shape in place (op:
+// DIAG_EVOLVE or DIAG_CONTRACT): specs holds one operator per state, or one
+// for all of them.  The column terms of each distinct operator, then one
+// elementwise launch with grid.z over the states; each state goes through
+// the same arithmetic as diag_impl's.
+void diag_batched_impl(Context& ctx, int op, int n_states, void* const* vecs, int dtype,
+                       int norb, int nrow_occ, int ncol_occ, const std::vector<DiagSpec>& specs,
+                       hipStream_t stream) {
   const long n_rows = (long)binom(norb, nrow_occ), n_cols = (long)binom(norb, ncol_occ);
   if (n_rows == 0 || n_cols == 0 || n_states == 0) return;
+  const bool shared = specs.size() == 1;
+  require(shared || (int)specs.size() == n_states, FFSIM_ERR_VALUE,
+          "batched diagonal operator: one operator per state");
+  require(op == DIAG_EVOLVE || op == DIAG_CONTRACT, FFSIM_ERR_VALUE, "unsupported operation");
   require(norb <= 63, FFSIM_ERR_VALUE, "norb > 63 unsupported");
-  require(spec.norb == norb, FFSIM_ERR_VALUE, "diagonal operator of another norb");
+  for (const DiagSpec& sp : specs)
+    require(sp.norb == norb, FFSIM_ERR_VALUE, "diagonal operator of another norb");
   require(diag_grid(n_rows, n_cols).y <= 65535, FFSIM_ERR_VALUE, "too many columns");
   const DevStrings& rs = dev_strings(ctx, norb, nrow_occ);
   const DevStrings& cs = dev_strings(ctx, norb, ncol_occ);
+  const int evolve = op == DIAG_EVOLVE ? 1 : 0;
+  const long gamma_len = n_cols * (evolve ? 2 : 1);
   constexpr int kChunk = 256;
   for (int i0 = 0; i0 < n_states; i0 += kChunk) {
     const int m = std::min(kChunk, n_states - i0);
+    const int n_ops = shared ? 1 : m;
     Program prog(ctx, stream);
-    // one header per state (the kernel indexes them by state), one coefficient set
-    const std::vector<DiagHeader> hs(m, diag_header(spec));
+    // one header per state (the kernel indexes them by state), coefficients
+    // and column terms per distinct operator
+    std::vector<DiagHeader> hs(m);
+    std::vector<double> coef;
+    for (int i = 0; i < m; ++i) {
+      const DiagSpec& sp = specs[shared ? 0 : i0 + i];
+      hs[i] = diag_header(sp);
+      if (i < n_ops) {
+        const std::vector<double> c = diag_coefficients(sp);
+        coef.insert(coef.end(), c.begin(), c.end());
+      }
+    }
+    const long coef_len = (long)coef.size() / n_ops;
     const size_t hoff = prog.append(hs.data(), hs.size() * sizeof(DiagHeader));
-    const std::vector<double> coef = diag_coefficients(spec);
     const size_t coff = prog.append(coef.data(), coef.size() * sizeof(double));
     std::vector<const void*> ptrs(vecs + i0, vecs + i0 + m);
     const size_t soff = prog.append(ptrs.data(), m * sizeof(void*));
     size_t goff = 0;
-    require(prog.reserve((size_t)n_cols * 2 * sizeof(double), &goff), FFSIM_ERR_VALUE,
+    require(prog.reserve((size_t)n_ops * gamma_len * sizeof(double), &goff), FFSIM_ERR_VALUE,
             "diagonal operator: too many columns");
     prog.commit();
     const DiagHeader* dh = prog.dev<DiagHeader>(hoff);
     const double* dc = prog.dev<double>(coff);
     double* dg = prog.scratch<double>(goff);
-    launch_diag_col_term(n_cols, cs.d, dh, dc, dg, 1, stream);
+    const long cstride = shared ? 0 : coef_len, gstride = shared ? 0 : gamma_len;
+    launch_diag_col_term(n_cols, cs.d, dh, dc, dg, evolve, stream, n_ops, cstride, gstride);
     HIP_CHECK(hipGetLastError());
     const double moved = 2.0 * (double)elem_bytes(dtype) * (double)n_rows * (double)n_cols * m;
     const long pk = prof_start(PROF_DIAG, moved, stream);
     if (dtype == FFSIM_C128)
-      launch_diag_pass_t<double>(DIAG_EVOLVE, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg,
-                                 dh, dc, stream, prog.dev<void*>(soff), m, 0, 0);
+      launch_diag_pass_t<double>(op, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg, dh, dc,
+                                 stream, prog.dev<void*>(soff), m, cstride, gstride);
     else
-      launch_diag_pass_t<float>(DIAG_EVOLVE, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg,
-                                dh, dc, stream, prog.dev<void*>(soff), m, 0, 0);
+      launch_diag_pass_t<float>(op, nullptr, nullptr, n_rows, n_cols, rs.d, cs.d, dg, dh, dc,
+                                stream, prog.dev<void*>(soff), m, cstride, gstride);
     HIP_CHECK(hipGetLastError());
     prof_stop(pk, stream);
     prog.finish();
@@ -2143,8 +2164,8 @@ int ffsim_apply_diag_coulomb_evolution_batched(int n_states, void* const* vecs, 
     Context& ctx = context();
     DiagSpec s = diag_coulomb_spec(norb, mat_aa, mat_ab, mat_bb, z_representation, time);
     if (!s.flags) return;
-    diag_evolve_batched_impl(ctx, n_states, vecs, dtype, norb, n_alpha, n_beta, s,
-                             (hipStream_t)stream);
+    diag_batched_impl(ctx, DIAG_EVOLVE, n_states, vecs, dtype, norb, n_alpha, n_beta, {s},
+                      (hipStream_t)stream);
   });
 }
 
@@ -2177,6 +2198,32 @@ int ffsim_contract_diag_coulomb(const void* vec, void* out, int dtype, int norb,
     DiagSpec s = diag_coulomb_spec(norb, mat_aa, mat_ab, mat_bb, z_representation, scale);
     diag_impl(ctx, accumulate ? DIAG_CONTRACT_ACC : DIAG_CONTRACT, vec, out, dtype, norb,
               n_alpha, n_beta, s, (hipStream_t)stream);
+  });
+}
+
+int ffsim_contract_diag_coulomb_batched(int n_states, void* const* vecs, int dtype, int norb,
+                                        int n_alpha, int n_beta, const double* mats_aa,
+                                        const double* mats_ab, const double* mats_bb,
+                                        int z_representation, double scale, void* stream) {
+  return guarded([&] {
+    check_dtype(dtype);
+    check_sizes(norb, n_alpha, n_beta);
+    require(n_states >= 0, FFSIM_ERR_VALUE, "negative batch size");
+    if (n_states == 0) return;
+    require(vecs != nullptr, FFSIM_ERR_VALUE, "batched contraction: no state array");
+    for (int i = 0; i < n_states; ++i)
+      require(vecs[i] != nullptr, FFSIM_ERR_VALUE, "batched contraction: null state");
+    Context& ctx = context();
+    const size_t nn = (size_t)norb * norb;
+    std::vector<DiagSpec> specs;
+    specs.reserve(n_states);
+    for (int i = 0; i < n_states; ++i)
+      specs.push_back(diag_coulomb_spec(norb, mats_aa ? mats_aa + i * nn : nullptr,
+                                        mats_ab ? mats_ab + i * nn : nullptr,
+                                        mats_bb ? mats_bb + i * nn : nullptr, z_representation,
+                                        scale));
+    diag_batched_impl(ctx, DIAG_CONTRACT, n_states, vecs, dtype, norb, n_alpha, n_beta, specs,
+                      (hipStream_t)stream);
   });
 }
 

@@ -12,7 +12,10 @@ import torch
 from ffsim_amd import _device, _lib
 from ffsim_amd import linalg as _linalg
 from ffsim_amd._linop import DeviceLinearOperator
-from ffsim_amd.contract.diag_coulomb import contract_diag_coulomb_into
+from ffsim_amd.contract.diag_coulomb import (
+    contract_diag_coulomb_into,
+    contract_diag_coulombs_in_place,
+)
 from ffsim_amd.contract.num_op_sum import contract_num_op_sum_into
 from ffsim_amd.gates.orbital_rotation import orbital_rotation_in_place, orbital_rotations_in_place
 
@@ -179,18 +182,19 @@ class DoubleFactorizedHamiltonian:
             result = torch.empty_like(v)
             axpby_(result, v, None, constant)
             if _batch_terms(v, norb, nelec):
-                # small states (no transposition buffer): one copy per term, and
-                # the terms' rotations share their launches (the same
-                # arithmetic per term as the loop below, in the same order)
+                # small states (no transposition buffer): one copy per term; the
+                # terms' rotations and contractions share their launches (the
+                # same arithmetic per term as the loop below, summed in the
+                # same order)
                 ts = [v.clone() for _ in range(1 + len(mats))]
                 orbital_rotations_in_place(
                     ts, [vecs.T.conj()] + [rot.T.conj() for rot in rots], norb, nelec)
                 contract_num_op_sum_into(ts[0], ts[0], eigs, norb, nelec)
-                for t, mat in zip(ts[1:], mats):
-                    contract_diag_coulomb_into(t, t, mat, norb, nelec, z_rep)
+                if mats:
+                    contract_diag_coulombs_in_place(ts[1:], mats, norb, nelec, z_rep)
                 orbital_rotations_in_place(ts, [vecs] + rots, norb, nelec)
                 for t in ts:
-                    axpby_(result, result, t, 1.0, 1.0)
+                    result.add_(t)
                 return result
             t = torch.empty_like(v)
             t.copy_(v)

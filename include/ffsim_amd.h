@@ -149,6 +149,19 @@ int ffsim_contract_diag_coulomb(const void* vec, void* out, int dtype, int norb,
                                 const double* mat_bb, int z_representation, double scale,
                                 int accumulate, void* stream);
 
+/* ffsim_contract_diag_coulomb in place on n_states states of one shape, each
+ * with its own matrices (mats_*: NULL or n_states x norb x norb, in the
+ * convention of ffsim_contract_diag_coulomb), in one elementwise launch:
+ * vec_i = scale * E_i(a, b) * vec_i.  Batching has no counterpart in the
+ * reference, whose DoubleFactorizedHamiltonian._linear_operator_ contracts
+ * its terms one by one (python/ffsim/hamiltonians/
+ * double_factorized_hamiltonian.py:244-275).  Each result is bitwise equal to
+ * ffsim_contract_diag_coulomb's (with vec == out). */
+int ffsim_contract_diag_coulomb_batched(int n_states, void* const* vecs, int dtype, int norb,
+                                        int n_alpha, int n_beta, const double* mats_aa,
+                                        const double* mats_ab, const double* mats_bb,
+                                        int z_representation, double scale, void* stream);
+
 /* Replaces _lib.contract_num_op_sum_spin_into_buffer applied to both spin
  * sectors (src/contract/num_op_sum.rs:18-39, python/ffsim/contract/num_op_sum.py:27-72):
  * out (=|+=) scale * (sum_{i in a} ca[i] + sum_{j in b} cb[j]) vec[a,b]. */

@@ -264,3 +264,21 @@ def test_batched_diag_argument_errors_without_gpu():
     with pytest.raises(ValueError, match="negative"):
         _lib.check(_lib.lib().ffsim_apply_diag_coulomb_evolution_batched(
             -1, None, _lib.C128, norb, 2, 2, None, None, None, 1.0, 0, None))
+
+
+def test_batched_contraction_argument_errors_without_gpu():
+    """The batched contraction checks its batch before touching the device."""
+    import torch
+
+    from ffsim_amd.contract.diag_coulomb import contract_diag_coulombs_in_place
+
+    norb, nelec = 4, (2, 2)
+    states = [torch.zeros(36, dtype=torch.complex128) for _ in range(2)]
+    j = np.eye(norb)
+    with pytest.raises(ValueError, match="one operator per state"):
+        contract_diag_coulombs_in_place(states, [j], norb, nelec)
+    with pytest.raises(ValueError, match="same interactions"):
+        contract_diag_coulombs_in_place(states, [(j, None, j), (j, j, j)], norb, nelec)
+    with pytest.raises(ValueError, match="negative"):
+        _lib.check(_lib.lib().ffsim_contract_diag_coulomb_batched(
+            -1, None, _lib.C128, norb, 2, 2, None, None, None, 0, 1.0, None))

@@ -1379,11 +1379,12 @@ void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int
 // launches are when all states share them.  Measured per state (LUCJ
 // n_reps=2, complex128, MI355X; GEMM vs Givens with all states per launch):
 // norb=8 (70 strings) 0.29 vs 0.57 ms alone and 0.014 vs 0.031 ms in a batch
-// of 32; norb=10 (252 strings) 0.37 vs 0.32 ms alone and 0.073 vs 0.061 ms in
-// a batch of 32; norb=12 (924 strings) 2.6 vs 0.20 ms in a batch of 32.  The
-// dim^3 against dim^2 scaling puts the crossover in batches near 150 strings.
+// of 32; norb=9 (126 strings) 0.055 vs 0.078 ms in a batch of 8 and 0.023 vs
+// 0.042 in a batch of 32; norb=10 (4,4) (210 strings) 0.070 vs 0.071 and
+// 0.056 vs 0.055; norb=10 (5,5) (252 strings) 0.076 vs 0.076 and 0.070 vs
+// 0.056; norb=12 (924 strings) 2.6 vs 0.20 ms in a batch of 32.
 constexpr long kGemmAlwaysDim = 100;   // any batch
-constexpr long kGemmBatchDim = 150;    // batches of at least kGemmMinBatch states
+constexpr long kGemmBatchDim = 200;    // batches of at least kGemmMinBatch states
 constexpr int kGemmMinBatch = 8;
 
 // FFSIM_AMD_BATCH_PATH=gemm | sweep | states forces a path (read at every

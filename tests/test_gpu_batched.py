@@ -290,3 +290,36 @@ def test_batched_diag_coulomb_contraction_bitwise(z_rep, dtype):
         np.testing.assert_allclose(got, expected, atol=1e-12, rtol=0)
     else:
         assert np.abs(got - expected).max() <= 2e-6 * np.abs(expected).max()
+
+
+@pytest.mark.parametrize("order", ["clements", "triangle"])
+def test_batched_multi_state_both_givens_orders(order, monkeypatch):
+    """The multi-state launches under either Givens order (forced with
+    FFSIM_AMD_GIVENS_ORDER): bitwise the single-state results, and the
+    oracle's to 1e-12."""
+    import torch
+
+    import ffsim_amd
+    from ffsim_amd.variational.ucj_spin_balanced import UCJOpSpinBalanced
+
+    norb, nelec, n_states = 10, (5, 4), 6
+    monkeypatch.setenv("FFSIM_AMD_GIVENS_ORDER", order)
+    monkeypatch.setenv("FFSIM_AMD_BATCH_PATH", "sweep")
+    rng = np.random.default_rng(77)
+    _case("batched ucj order", order, norb, nelec, n_states)
+    ops = [ffsim_amd.random.random_ucj_op_spin_balanced(
+        norb, n_reps=2, interaction_pairs=_pairs(norb), with_final_orbital_rotation=True,
+        seed=rng) for _ in range(n_states)]
+    dim = math.comb(norb, nelec[0]) * math.comb(norb, nelec[1])
+    vecs = [O.random_state_vector(dim, rng) for _ in range(n_states)]
+    states = [torch.from_numpy(v).to("cuda") for v in vecs]
+    single = [s.clone() for s in states]
+    UCJOpSpinBalanced.apply_batched_in_place(ops, states, norb, nelec)
+    for op, s in zip(ops, single):
+        op.apply_in_place(s, norb, nelec)
+    for got, one in zip(states, single):
+        assert torch.equal(got, one)
+    op = ops[-1]
+    expected = O.apply_ucj_spin_balanced(vecs[-1], op.diag_coulomb_mats, op.orbital_rotations,
+                                         op.final_orbital_rotation, norb, nelec)
+    np.testing.assert_allclose(states[-1].cpu().numpy(), expected, atol=1e-12, rtol=0)

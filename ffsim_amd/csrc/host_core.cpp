@@ -553,8 +553,62 @@ std::vector<int> cached_beam_starts(const std::vector<KStep>& steps, int n, int 
 }
 }  // namespace
 
+namespace {
+// The sub-pass plan of a step list depends only on the steps' pair positions;
+// plans are kept per position sequence with each used slot holding the index
+// of its step, and filled with the coefficients of each call.
+std::mutex g_sub_mu;
+std::map<std::vector<int>, std::pair<std::vector<SubPass>, std::vector<LayerPair>>> g_sub_cache;
+
+void plan_sub_passes_uncached(const std::vector<KStep>& window_steps, int w, int w2,
+                              std::vector<SubPass>& subs, std::vector<LayerPair>& layers);
+}  // namespace
+
 void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
                      std::vector<SubPass>& subs, std::vector<LayerPair>& layers) {
+  subs.clear();
+  layers.clear();
+  if (window_steps.empty()) return;
+  std::vector<int> key;
+  key.reserve(window_steps.size() + 2);
+  key.push_back(w);
+  key.push_back(w2);
+  for (const KStep& k : window_steps) key.push_back(k.p);
+  bool found = false;
+  {
+    std::lock_guard<std::mutex> lk(g_sub_mu);
+    auto it = g_sub_cache.find(key);
+    if (it != g_sub_cache.end()) {
+      subs = it->second.first;
+      layers = it->second.second;
+      found = true;
+    }
+  }
+  if (!found) {
+    std::vector<KStep> proxy(window_steps);
+    for (size_t r = 0; r < proxy.size(); ++r) {
+      proxy[r].c = (double)r;
+      proxy[r].sr = 0.0;
+      proxy[r].si = 0.0;
+    }
+    plan_sub_passes_uncached(proxy, w, w2, subs, layers);
+    std::lock_guard<std::mutex> lk(g_sub_mu);
+    if (g_sub_cache.size() > 4096) g_sub_cache.clear();
+    g_sub_cache[key] = {subs, layers};
+  }
+  for (LayerPair& L : layers)
+    for (int q = 0; q < kSlots; ++q)
+      if ((L.mask >> q) & 1) {
+        const KStep& k = window_steps[(size_t)L.c[q]];
+        L.c[q] = k.c;
+        L.sr[q] = k.sr;
+        L.si[q] = k.si;
+      }
+}
+
+namespace {
+void plan_sub_passes_uncached(const std::vector<KStep>& window_steps, int w, int w2,
+                              std::vector<SubPass>& subs, std::vector<LayerPair>& layers) {
   subs.clear();
   layers.clear();
   if (window_steps.empty()) return;
@@ -618,6 +672,7 @@ void plan_sub_passes(const std::vector<KStep>& window_steps, int w, int w2,
     subs.push_back(s);
   }
 }
+}  // namespace
 
 // ------------------------------------------------------------------ Givens
 namespace {

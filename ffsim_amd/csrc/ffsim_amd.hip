@@ -1378,13 +1378,13 @@ void ucj_impl(Context& ctx, void* vec, void* work, int dtype, int n, int na, int
 // batched GEMMs per rotation) is shared by the batch, as the Givens path's
 // launches are when all states share them.  Measured per state (LUCJ
 // n_reps=2, complex128, MI355X; GEMM vs Givens with all states per launch):
-// norb=8 (70 strings) 0.29 vs 0.57 ms alone and 0.014 vs 0.031 ms in a batch
-// of 32; norb=9 (126 strings) 0.055 vs 0.078 ms in a batch of 8 and 0.023 vs
-// 0.042 in a batch of 32; norb=10 (4,4) (210 strings) 0.070 vs 0.071 and
-// 0.056 vs 0.055; norb=10 (5,5) (252 strings) 0.076 vs 0.076 and 0.070 vs
-// 0.056; norb=12 (924 strings) 2.6 vs 0.20 ms in a batch of 32.
+// norb=8 (70 strings) 0.26 vs 0.54 ms alone, 0.051 vs 0.067 ms in a batch of
+// 8 and 0.017 vs 0.024 in a batch of 32; norb=9 (126 strings) 0.052 vs 0.061
+// and 0.021 vs 0.029; norb=10 (4,4) (210 strings) 0.067 vs 0.047 and 0.053 vs
+// 0.033; norb=10 (5,5) (252 strings) 0.077 vs 0.051 and 0.069 vs 0.033;
+// norb=12 (924 strings) 2.6 vs 0.16 ms in a batch of 32.
 constexpr long kGemmAlwaysDim = 100;   // any batch
-constexpr long kGemmBatchDim = 200;    // batches of at least kGemmMinBatch states
+constexpr long kGemmBatchDim = 150;    // batches of at least kGemmMinBatch states
 constexpr int kGemmMinBatch = 8;
 
 // FFSIM_AMD_BATCH_PATH=gemm | sweep | states forces a path (read at every

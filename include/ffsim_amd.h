@@ -193,7 +193,7 @@ int ffsim_apply_ucj_spin_balanced(void* vec, void* work, int dtype, int norb, in
  * taken).  diag_coulomb_mats: n_states x n_reps x 2 x norb x norb;
  * orbital_rotations: n_states x n_reps x norb x norb complex128;
  * final_orbital_rotations: NULL or n_states x norb x norb.  Small states
- * (at most 6 electrons per sector and at most 100 strings, or 200 strings in
+ * (at most 6 electrons per sector and at most 100 strings, or 150 strings in
  * batches of 8 or more) take a GEMM path: each merged orbital rotation is the
  * compound matrix Lambda^k(U) (built on the device) applied to all states by
  * batched complex GEMMs, and each diagonal term one batched elementwise pass.
